@@ -1,0 +1,190 @@
+// Per-chain device state and the launch wrappers every kernel file exports.
+//
+// A "batch" is an array of ChainDesc in device memory; kernels index it with
+// blockIdx.y so one launch advances every chain of the batch. A single chain
+// is a batch of one. Matrices are n x n complex128, column-major (Julia
+// ComplexF64 layout), leading dimension n.
+#pragma once
+#include "device_common.hpp"
+
+namespace dqmc {
+
+// Named n x n complex work matrices of a chain.
+enum MatId : int {
+  M_G = 0,     // equal-time Green's function (mc.s.greens)
+  M_GW,        // wrapped copy for the drift check (mc.s.greens_temp)
+  M_GALT,      // ping-pong partner of G inside the local-update kernel
+  M_UL, M_TL,  // left UDT operand of calculate_greens
+  M_UR, M_TR,  // right UDT operand
+  M_TMP, M_TMP2, M_TMP3, M_TMP4,
+  M_CURU,      // curr_U of add_slice_sequence_*
+  M_QRA,       // factorization workspace (R and reflectors)
+  M_QRQ,       // explicit Q of the last factorization
+  M_QRT,       // T of the last factorization
+  M_UG, M_TG,  // s.U and s.T kept after calculate_greens (logdet)
+  M_IDENT,     // identity
+  M_GRAD,      // spare n x n for the gradient/TDGF paths
+  M_ET2,       // dense exp(-dtau T/2)^2        (no checkerboard; operator, read-only)
+  M_ETINV2,    // dense exp(+dtau T/2)^2
+  kNumMat
+};
+// Named length-n real vectors.
+enum VecId : int { V_DL = 0, V_DR, V_D1, V_D2, V_DG, V_ONES, V_QRD, kNumVec };
+
+enum MatKind : int { MK_NAMED = 0, MK_USTACK = 1, MK_TSTACK = 2 };
+struct MatRef {
+  int kind;
+  int idx;
+};
+enum VecKind : int { VK_NAMED = 0, VK_DSTACK = 1 };
+struct VecRef {
+  int kind;
+  int idx;
+};
+
+struct ChainDesc {
+  // model
+  int L, N, n, M, safe_mult, nelem, chkr;  // chkr: 1 = Assaad checkerboard, 0 = dense hopping
+  int edrun;
+  double dtau, lam, r, c, u;
+  // operators (host-computed at setup, read-only)
+  const int* plaq_sites;  // [2 groups][N/4][4] site indices
+  const cplx* plaq;       // [4 variants][2 groups][4 blocks][N/4][16] row-major 4x4
+  const double* expmu;    // [n] exp(+dtau mu)   (chkr_mu diagonal)
+  const double* expmu_inv;
+  const int* nb;          // [4][N] neighbours (up, right, down, left)
+  // state
+  double* phi;            // (3, N, M) column-major
+  cplx* mat[kNumMat];
+  double* vec[kNumVec];
+  cplx* ustack;           // [nelem][n*n]
+  cplx* tstack;           // [nelem][n*n]
+  double* dstack;         // [nelem][n]
+  // factorization scratch
+  double* qr_vn;          // [2n] partial column norms
+  cplx* qr_cand;          // [2][W][n] candidate columns (double buffered)
+  double* qr_cand_norm;   // [2][W]
+  int* qr_cand_pos;       // [2][W]
+  int* qr_perm;           // [n] logical -> physical column
+  int* qr_lpos;           // [n] physical -> logical
+  cplx* qr_tau;           // [n]
+  int* inv_piv;           // [n]
+  cplx* inv_bcast;        // [2][n]
+  cplx* lu_small;         // [32] delta (16) and M^-1 (16) of the last single proposal
+  // local updates
+  const double* uniforms; // host-drawn uniform stream (device copy)
+  long long* ucursor;     // next unread uniform
+  long long n_uniforms;
+  double* scal;           // per-chain scalars, see ScalarId
+  double* prop_log;       // [4 * log_cap] per-proposal (p_acc, u or -1, accepted, detratio imag)
+  long long log_cap;
+  long long* log_count;
+  // group barrier
+  unsigned int* bar_count;
+  unsigned int* bar_gen;
+  int* error;
+};
+
+// Per-chain scalar slots (ChainDesc::scal).
+enum ScalarId : int {
+  S_BOSON_ACTION = 0,  // p.boson_action
+  S_ACC_SLICE,         // accepted proposals in the last local_updates call
+  S_ACC_TOTAL,         // accepted proposals since reset
+  S_PROP_TOTAL,        // proposals since reset
+  S_DRIFT_LAST,        // max|G_wrapped - G_fresh| of the last recompute
+  S_DRIFT_MAX,         // running max of the above
+  S_LOGDET,            // log|det G| (calculate_logdet)
+  S_LOGDET_A,          // log|det| scratch from inversions
+  S_NONREAL,           // proposals with |Im r / Re r| > 1e-4
+  S_MINRATIO,          // min |p_acc - u| over proposals that drew u
+  kNumScalar = 16
+};
+
+__device__ inline cplx* matptr(const ChainDesc& c, MatRef r) {
+  const size_t nn = (size_t)c.n * c.n;
+  if (r.kind == MK_NAMED) return c.mat[r.idx];
+  if (r.kind == MK_USTACK) return c.ustack + nn * r.idx;
+  return c.tstack + nn * r.idx;
+}
+__device__ inline double* vecptr(const ChainDesc& c, VecRef r) {
+  if (r.kind == VK_NAMED) return c.vec[r.idx];
+  return c.dstack + (size_t)c.n * r.idx;
+}
+
+__device__ inline GroupSync chain_group(const ChainDesc& c) {
+  return GroupSync{c.bar_count, c.bar_gen, c.error, gridDim.x};
+}
+
+}  // namespace dqmc
+
+// ---------------------------------------------------------------------------
+// Host launch wrappers (each defined next to its kernel). All are asynchronous
+// on `stream`; `chains` points to `nchains` device-resident ChainDesc entries
+// and `nmax` is the largest n in the batch.
+// ---------------------------------------------------------------------------
+namespace dqmc {
+
+enum BOp : int { B_LEFT = 0, B_RIGHT, B_INV_LEFT, B_INV_RIGHT, B_DAG_LEFT };
+
+// X <- op(B_slice) X  or  X op(B_slice) with the checkerboard operators; slice is 0-based
+// and the same for all chains, or taken per chain from `slices` when non-null.
+hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                               BOp op, int slice, const int* slices);
+// only the eV factor: mode 0 N, 1 T, 2 C; rows = apply to rows (from the right)
+hipError_t launch_apply_eV(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                           bool rows, int mode, double power, int slice, const int* slices);
+
+enum GemmOp : int { OP_N = 0, OP_C = 1 };
+// C = diag(dl) op(A) op(B) diag(dr)  (+ beta C); dl/dr optional (kind < 0 disables)
+hipError_t launch_gemm(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef C,
+                       MatRef A, GemmOp opA, MatRef B, GemmOp opB, VecRef dl, VecRef dr, double beta);
+
+// elementwise helpers
+hipError_t launch_copy(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef dst, MatRef src);
+hipError_t launch_copy_vec(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef dst, VecRef src);
+hipError_t launch_identity(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef dst, VecRef ones);
+hipError_t launch_ones_vec(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef dst);
+// dst = diag(d) src
+hipError_t launch_scale_rows_copy(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef dst,
+                                  MatRef src, VecRef d);
+hipError_t launch_scale_cols(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, VecRef d);
+hipError_t launch_add_diag(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, VecRef d);
+hipError_t launch_recip_vec(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef dst, VecRef src);
+hipError_t launch_adjoint(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef dst, MatRef src);
+// scal[S_DRIFT_LAST] = max|A - B|, scal[S_DRIFT_MAX] = max(that, old)
+hipError_t launch_maxdiff(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B);
+
+// A = Q diag(D) T with column-pivoted Householder QR: Q -> Qout, D -> vec, T -> Tout.
+// A is consumed (overwritten by the factorization). wg = workgroups per chain.
+hipError_t launch_udt(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
+                      MatRef Qout, MatRef Tout, int wg);
+
+// X <- X^-1 in place (Gauss-Jordan, partial pivoting); scal[S_LOGDET_A] = log|det X|.
+hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);
+
+// One local-update pass over all sites at `slice` (0-based).
+hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
+                                const int* slices, double box, int wg, int log);
+
+// scal[S_LOGDET] = log|det UG| + sum log DG + log|det TG|  helper pieces
+hipError_t launch_sum_log(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef d, int slot,
+                          int accumulate);
+hipError_t launch_add_scalar(hipStream_t s, const ChainDesc* chains, int nchains, int dst, int src);
+hipError_t launch_negate_scalar(hipStream_t s, const ChainDesc* chains, int nchains, int slot);
+
+// single proposal (calc_detratio) and forced update_greens! for unit use
+hipError_t launch_detratio(hipStream_t s, const ChainDesc* chains, int slice, int site, const double* new_op,
+                           double* out);
+hipError_t launch_force_update(hipStream_t s, const ChainDesc* chains, int nmax, int site);
+// reset counters/scratch of the per-chain scalars (keeps the boson action)
+hipError_t launch_reset_stats(hipStream_t s, const ChainDesc* chains, int nchains);
+
+// boson action of the current field into scal[S_BOSON_ACTION]
+hipError_t launch_boson_action(hipStream_t s, const ChainDesc* chains, int nchains, int Nmax);
+
+hipError_t launch_boson_grad(hipStream_t s, const ChainDesc* chains, int nchains, int NMmax, double* const* grad);
+// dS/dphi fermion part at one slice from G (slice 0-based): grad[3][N] += -Re tr[(1-G)_ii eV^-1 d eV]
+hipError_t launch_grad_slice(hipStream_t s, const ChainDesc* chains, int nchains, int Nmax, int slice,
+                             double* const* grad);
+
+}  // namespace dqmc

@@ -1,0 +1,137 @@
+// Device-side building blocks shared by every DQMC kernel: complex FP64
+// arithmetic, wave/block reductions, and the inter-workgroup barrier used by a
+// "chain group" (the set of workgroups that cooperate on one Markov chain).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dqmc {
+
+struct cplx {
+  double re, im;
+};
+
+__host__ __device__ inline cplx mk(double r, double i) { return cplx{r, i}; }
+__host__ __device__ inline cplx operator+(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cplx operator-(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cplx operator-(cplx a) { return cplx{-a.re, -a.im}; }
+__host__ __device__ inline cplx operator*(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline cplx operator*(double s, cplx a) { return cplx{s * a.re, s * a.im}; }
+__host__ __device__ inline cplx conj(cplx a) { return cplx{a.re, -a.im}; }
+__host__ __device__ inline double abs2(cplx a) { return a.re * a.re + a.im * a.im; }
+__host__ __device__ inline cplx& operator+=(cplx& a, cplx b) {
+  a.re += b.re;
+  a.im += b.im;
+  return a;
+}
+__host__ __device__ inline cplx& operator-=(cplx& a, cplx b) {
+  a.re -= b.re;
+  a.im -= b.im;
+  return a;
+}
+// a += b*c
+__host__ __device__ inline void cfma(cplx& a, cplx b, cplx c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(-b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(b.im, c.re, a.im);
+}
+// a += conj(b)*c
+__host__ __device__ inline void cfma_cj(cplx& a, cplx b, cplx c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(-b.im, c.re, a.im);
+}
+// 1/a (Smith's algorithm, avoids overflow in |a|^2)
+__host__ __device__ inline cplx cinv(cplx a) {
+  if (fabs(a.re) >= fabs(a.im)) {
+    double r = a.im / a.re, d = a.re + a.im * r;
+    return cplx{1.0 / d, -r / d};
+  }
+  double r = a.re / a.im, d = a.re * r + a.im;
+  return cplx{r / d, -1.0 / d};
+}
+
+// ---------------------------------------------------------------------------
+// Reductions (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline cplx wave_sum(cplx v) {
+  v.re = wave_sum(v.re);
+  v.im = wave_sum(v.im);
+  return v;
+}
+
+// Block-wide sum; `red` is LDS scratch of >= blockDim.x/64 doubles (x2 for cplx).
+__device__ inline double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < nw; ++i) t += red[i];
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Chain-group barrier
+//
+// A chain group is a set of `nblocks` workgroups cooperating on one chain,
+// guaranteed co-resident by a cooperative launch. The barrier is a generation
+// counter in device memory (agent-scope atomics, release/acquire fences).
+// Every wait is bounded: when the spin limit is hit the group's error word is
+// set and every later barrier returns immediately, so a logic error ends the
+// kernel instead of hanging the device.
+// ---------------------------------------------------------------------------
+struct GroupSync {
+  unsigned int* count;  // arrivals in the current generation
+  unsigned int* gen;    // generation counter
+  int* error;           // nonzero once a wait timed out
+  unsigned int nblocks;
+};
+
+constexpr long long kSpinLimit = 1ll << 26;  // ~ seconds of polling before giving up
+
+__device__ inline bool group_failed(const GroupSync& g) {
+  return __hip_atomic_load(g.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+__device__ inline void group_sync(const GroupSync& g) {
+  if (g.nblocks <= 1) {
+    __syncthreads();
+    return;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int my_gen = __hip_atomic_load(g.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned int arrived =
+        __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (arrived == g.nblocks) {
+      __hip_atomic_store(g.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(g.gen, my_gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long long spins = 0;
+      while (__hip_atomic_load(g.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
+        if (__hip_atomic_load(g.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (++spins > kSpinLimit) {
+          __hip_atomic_store(g.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+}  // namespace dqmc

@@ -1,0 +1,1138 @@
+// C ABI and host orchestration of the DQMC sweep path.
+//
+// The host side mirrors the reference's control flow (stack.jl: build_stack,
+// add_slice_sequence_left/right, calculate_greens, calculate_logdet,
+// wrap_greens!, propagate; local_updates.jl: local_updates) and enqueues the
+// device kernels on one HIP stream per shape group; no call synchronises unless
+// it returns data to the host. A single-chain context is a batch of one chain.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/dqmc_hip.h"
+#include "chain.hpp"
+
+using namespace dqmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t _e = (expr);                                                                           \
+    if (_e != hipSuccess)                                                                             \
+      return fail(DQMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));                     \
+  } while (0)
+
+constexpr int kMaxWG = 128;
+
+MatRef NM(int id) { return MatRef{MK_NAMED, id}; }
+MatRef US(int k) { return MatRef{MK_USTACK, k}; }
+MatRef TS(int k) { return MatRef{MK_TSTACK, k}; }
+VecRef NV(int id) { return VecRef{VK_NAMED, id}; }
+VecRef DS(int k) { return VecRef{VK_DSTACK, k}; }
+const VecRef NOVEC{-1, 0};
+
+struct ChainMem {
+  dqmc_params p;
+  int L, N, n, M, nelem;
+  ChainDesc d;  // host copy of the device descriptor
+  std::vector<void*> allocs;
+  double* uniforms = nullptr;
+  long long ucap = 0;
+  double* log_buf = nullptr;
+  long long log_cap = 0;
+};
+
+struct Group {
+  int n, N, L, M, s, nelem, chkr;
+  bool all_checks;
+  std::vector<int> chains;  // indices into Batch::chains
+  ChainDesc* d_desc = nullptr;
+  hipStream_t stream = nullptr;
+  int cur = 0, dir = 1;  // reference current_slice (1-based), direction
+  int wg_qr = 1, wg_inv = 1, wg_local = 1;
+  bool recomputed = false;
+  int nch() const { return (int)chains.size(); }
+};
+
+}  // namespace
+
+struct dqmc_batch {
+  int device = 0;
+  std::vector<ChainMem> chains;
+  std::vector<Group> groups;
+  std::vector<int> chain_group;  // chain -> group index
+  double* d_scratch = nullptr;   // small host-visible staging (detratio etc.)
+  bool use_graph = false;
+};
+
+struct dqmc_ctx {
+  dqmc_batch* b;
+};
+
+namespace {
+
+int alloc(ChainMem& cm, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes == 0 ? 16 : bytes);
+  if (e != hipSuccess) return fail(DQMC_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  cm.allocs.push_back(*p);
+  return 0;
+}
+
+int validate(const dqmc_params* p, const dqmc_operators* o) {
+  if (!p || !o) return fail(DQMC_E_ARG, "null params/operators");
+  if (p->L < 2 || p->slices < 1 || p->safe_mult < 1) return fail(DQMC_E_ARG, "bad L/slices/safe_mult");
+  if (p->slices % p->safe_mult != 0) return fail(DQMC_E_ARG, "slices must be a multiple of safe_mult");
+  if (4 * p->L * p->L > 1024) return fail(DQMC_E_ARG, "n = 4L^2 must be <= 1024");
+  if (p->checkerboard && (p->L % 2 != 0)) return fail(DQMC_E_ARG, "checkerboard needs even L");
+  if (!o->neighbors) return fail(DQMC_E_ARG, "neighbors required");
+  if (p->checkerboard && (!o->plaq_sites || !o->plaq || !o->expmu))
+    return fail(DQMC_E_ARG, "checkerboard operators required");
+  if (!p->checkerboard && (!o->hop_exp || !o->hop_exp_inv)) return fail(DQMC_E_ARG, "dense operators required");
+  return 0;
+}
+
+// complex n x n product on the host for setup only (exp(-dtau T/2)^2)
+std::vector<double> host_cmatmul(const double* A, const double* B, int n, bool adjA) {
+  std::vector<double> C((size_t)2 * n * n, 0.0);
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < n; ++k) {
+      const double br = B[2 * ((size_t)j * n + k)], bi = B[2 * ((size_t)j * n + k) + 1];
+      for (int i = 0; i < n; ++i) {
+        double ar, ai;
+        if (!adjA) {
+          ar = A[2 * ((size_t)k * n + i)];
+          ai = A[2 * ((size_t)k * n + i) + 1];
+        } else {
+          ar = A[2 * ((size_t)i * n + k)];
+          ai = -A[2 * ((size_t)i * n + k) + 1];
+        }
+        C[2 * ((size_t)j * n + i)] += ar * br - ai * bi;
+        C[2 * ((size_t)j * n + i) + 1] += ar * bi + ai * br;
+      }
+    }
+  return C;
+}
+
+int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
+  int rc = validate(p, o);
+  if (rc) return rc;
+  cm.p = *p;
+  cm.L = p->L;
+  cm.N = p->L * p->L;
+  cm.n = 4 * cm.N;
+  cm.M = p->slices;
+  cm.nelem = p->slices / p->safe_mult + 1;
+  const int n = cm.n, N = cm.N;
+  const size_t nn = (size_t)n * n;
+  ChainDesc& d = cm.d;
+  memset(&d, 0, sizeof(d));
+  d.L = cm.L;
+  d.N = N;
+  d.n = n;
+  d.M = cm.M;
+  d.safe_mult = p->safe_mult;
+  d.nelem = cm.nelem;
+  d.chkr = p->checkerboard;
+  d.edrun = p->edrun;
+  d.dtau = p->delta_tau;
+  d.lam = p->lambda;
+  d.r = p->r;
+  d.c = p->c;
+  d.u = p->u;
+  void* ptr;
+#define ALLOC(field, type, count)                                  \
+  do {                                                             \
+    if ((rc = alloc(cm, &ptr, sizeof(type) * (size_t)(count)))) return rc; \
+    field = (type*)ptr;                                            \
+  } while (0)
+  for (int m = 0; m < kNumMat; ++m) {
+    if ((m == M_ET2 || m == M_ETINV2) && p->checkerboard) continue;
+    ALLOC(d.mat[m], cplx, nn);
+  }
+  for (int v = 0; v < kNumVec; ++v) ALLOC(d.vec[v], double, n);
+  ALLOC(d.ustack, cplx, nn * cm.nelem);
+  ALLOC(d.tstack, cplx, nn * cm.nelem);
+  ALLOC(d.dstack, double, (size_t)n * cm.nelem);
+  ALLOC(d.phi, double, (size_t)3 * N * cm.M);
+  ALLOC(d.qr_vn, double, 2 * n);
+  ALLOC(d.qr_cand, cplx, (size_t)2 * kMaxWG * n);
+  ALLOC(d.qr_cand_norm, double, 2 * kMaxWG);
+  ALLOC(d.qr_cand_pos, int, 2 * kMaxWG);
+  ALLOC(d.qr_perm, int, n);
+  ALLOC(d.qr_lpos, int, n);
+  ALLOC(d.qr_tau, cplx, n);
+  ALLOC(d.inv_piv, int, n);
+  ALLOC(d.inv_bcast, cplx, 2 * n);
+  ALLOC(d.lu_small, cplx, 32);
+  ALLOC(d.ucursor, long long, 1);
+  ALLOC(d.scal, double, kNumScalar);
+  ALLOC(d.log_count, long long, 1);
+  unsigned int* bar;
+  ALLOC(bar, unsigned int, 2);
+  d.bar_count = bar;
+  d.bar_gen = bar + 1;
+  ALLOC(d.error, int, 1);
+  HIPCHK(hipMemset(bar, 0, 2 * sizeof(unsigned int)));
+  HIPCHK(hipMemset(d.error, 0, sizeof(int)));
+  HIPCHK(hipMemset(d.ucursor, 0, sizeof(long long)));
+  HIPCHK(hipMemset(d.log_count, 0, sizeof(long long)));
+  HIPCHK(hipMemset(d.phi, 0, sizeof(double) * 3 * N * cm.M));
+  {
+    std::vector<double> sc(kNumScalar, 0.0);
+    sc[S_MINRATIO] = 1e300;
+    HIPCHK(hipMemcpy(d.scal, sc.data(), sizeof(double) * kNumScalar, hipMemcpyHostToDevice));
+  }
+  // operators
+  int* nbd;
+  ALLOC(nbd, int, 4 * N);
+  HIPCHK(hipMemcpy(nbd, o->neighbors, sizeof(int) * 4 * N, hipMemcpyHostToDevice));
+  d.nb = nbd;
+  if (p->checkerboard) {
+    int* ps;
+    cplx* pl;
+    double *em, *emi;
+    ALLOC(ps, int, 2 * N);
+    ALLOC(pl, cplx, (size_t)4 * 2 * 4 * (N / 4) * 16);
+    ALLOC(em, double, n);
+    ALLOC(emi, double, n);
+    HIPCHK(hipMemcpy(ps, o->plaq_sites, sizeof(int) * 2 * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pl, o->plaq, sizeof(cplx) * 4 * 2 * 4 * (N / 4) * 16, hipMemcpyHostToDevice));
+    std::vector<double> inv(n);
+    for (int i = 0; i < n; ++i) inv[i] = 1.0 / o->expmu[i];
+    HIPCHK(hipMemcpy(em, o->expmu, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(emi, inv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    d.plaq_sites = ps;
+    d.plaq = pl;
+    d.expmu = em;
+    d.expmu_inv = emi;
+  } else {
+    // B = eT (eT eV): the product of the two half-step exponentials is formed once
+    std::vector<double> e2 = host_cmatmul(o->hop_exp, o->hop_exp, n, false);
+    std::vector<double> ei2 = host_cmatmul(o->hop_exp_inv, o->hop_exp_inv, n, false);
+    HIPCHK(hipMemcpy(d.mat[M_ET2], e2.data(), sizeof(cplx) * nn, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.mat[M_ETINV2], ei2.data(), sizeof(cplx) * nn, hipMemcpyHostToDevice));
+  }
+#undef ALLOC
+  return 0;
+}
+
+void chain_free(ChainMem& cm) {
+  for (void* p : cm.allocs) (void)hipFree(p);
+  cm.allocs.clear();
+  if (cm.uniforms) (void)hipFree(cm.uniforms);
+  if (cm.log_buf) (void)hipFree(cm.log_buf);
+  cm.uniforms = nullptr;
+  cm.log_buf = nullptr;
+}
+
+int upload_desc(dqmc_batch* b, Group& g) {
+  std::vector<ChainDesc> h;
+  for (int ci : g.chains) h.push_back(b->chains[ci].d);
+  if (!g.d_desc) HIPCHK(hipMalloc(&g.d_desc, sizeof(ChainDesc) * h.size()));
+  HIPCHK(hipMemcpy(g.d_desc, h.data(), sizeof(ChainDesc) * h.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int default_groups(Group& g) {
+  const int nch = g.nch();
+  int per = std::max(1, 256 / nch);
+  g.wg_qr = std::max(1, std::min({per, g.n / 8, kMaxWG}));
+  g.wg_inv = std::max(std::max(1, (g.n + 511) / 512), std::min({per, g.n / 8, kMaxWG}));
+  g.wg_local = std::max(1, std::min({per, g.n / 8, kMaxWG}));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Group operations (enqueue only)
+// ---------------------------------------------------------------------------
+#define LCHK(expr)                                                                             \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess) return fail(DQMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+struct Ops {
+  Group& g;
+  explicit Ops(Group& gr) : g(gr) {}
+  hipStream_t s() const { return g.stream; }
+  const ChainDesc* d() const { return g.d_desc; }
+  int nc() const { return g.nch(); }
+
+  int copy(MatRef dst, MatRef src) {
+    LCHK(launch_copy(s(), d(), nc(), g.n, dst, src));
+    return 0;
+  }
+  int copyv(VecRef dst, VecRef src) {
+    LCHK(launch_copy_vec(s(), d(), nc(), g.n, dst, src));
+    return 0;
+  }
+  int gemm(MatRef C, MatRef A, GemmOp oa, MatRef B, GemmOp ob, VecRef dl = NOVEC, VecRef dr = NOVEC) {
+    LCHK(launch_gemm(s(), d(), nc(), g.n, C, A, oa, B, ob, dl, dr, 0.0));
+    return 0;
+  }
+  int udt(MatRef A, VecRef D, MatRef Q, MatRef T) {
+    LCHK(launch_udt(s(), d(), nc(), g.n, A, D, Q, T, g.wg_qr));
+    return 0;
+  }
+  int inverse(MatRef X) {
+    LCHK(launch_inverse(s(), d(), nc(), g.n, X, g.wg_inv));
+    return 0;
+  }
+  // X <- op(B_slice) X / X op(B_slice); slice 0-based
+  int applyB(MatRef X, BOp op, int slice) {
+    if (g.chkr) {
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, op, slice, nullptr));
+      return 0;
+    }
+    // dense: B = eT2 eV, B^-1 = eV^-1 eTinv2, B^dagger = eV eT2^dagger
+    const MatRef T4 = NM(M_TMP4);
+    switch (op) {
+      case B_LEFT:
+        LCHK(launch_apply_eV(s(), d(), nc(), g.n, g.N, X, false, 0, 1.0, slice, nullptr));
+        LCHK(launch_gemm(s(), d(), nc(), g.n, T4, NM(M_ET2), OP_N, X, OP_N, NOVEC, NOVEC, 0.0));
+        return copy(X, T4);
+      case B_RIGHT:
+        LCHK(launch_gemm(s(), d(), nc(), g.n, T4, X, OP_N, NM(M_ET2), OP_N, NOVEC, NOVEC, 0.0));
+        LCHK(launch_apply_eV(s(), d(), nc(), g.n, g.N, T4, true, 1, 1.0, slice, nullptr));
+        return copy(X, T4);
+      case B_INV_LEFT:
+        LCHK(launch_gemm(s(), d(), nc(), g.n, T4, NM(M_ETINV2), OP_N, X, OP_N, NOVEC, NOVEC, 0.0));
+        LCHK(launch_apply_eV(s(), d(), nc(), g.n, g.N, T4, false, 0, -1.0, slice, nullptr));
+        return copy(X, T4);
+      case B_INV_RIGHT:
+        LCHK(launch_apply_eV(s(), d(), nc(), g.n, g.N, X, true, 1, -1.0, slice, nullptr));
+        LCHK(launch_gemm(s(), d(), nc(), g.n, T4, X, OP_N, NM(M_ETINV2), OP_N, NOVEC, NOVEC, 0.0));
+        return copy(X, T4);
+      case B_DAG_LEFT:
+        LCHK(launch_gemm(s(), d(), nc(), g.n, T4, NM(M_ET2), OP_C, X, OP_N, NOVEC, NOVEC, 0.0));
+        LCHK(launch_apply_eV(s(), d(), nc(), g.n, g.N, T4, false, 2, 1.0, slice, nullptr));
+        return copy(X, T4);
+    }
+    return fail(DQMC_E_ARG, "bad B op");
+  }
+
+  // wrap_greens! (stack.jl:316-325), slice 1-based as in the reference
+  int wrap(MatRef X, int slice, int direction) {
+    int rc;
+    if (direction == -1) {
+      if ((rc = applyB(X, B_INV_LEFT, slice - 2))) return rc;
+      return applyB(X, B_RIGHT, slice - 2);
+    }
+    if ((rc = applyB(X, B_LEFT, slice - 1))) return rc;
+    return applyB(X, B_INV_RIGHT, slice - 1);
+  }
+
+  int identity_elem(int k) {
+    int rc;
+    LCHK(launch_identity(s(), d(), nc(), g.n, US(k), NOVEC));
+    LCHK(launch_identity(s(), d(), nc(), g.n, TS(k), NOVEC));
+    LCHK(launch_ones_vec(s(), d(), nc(), g.n, DS(k)));
+    (void)rc;
+    return 0;
+  }
+
+  // build_stack (stack.jl:251-272)
+  int build_stack() {
+    int rc;
+    if ((rc = identity_elem(0))) return rc;
+    for (int i = 0; i < g.nelem - 1; ++i)
+      if ((rc = add_left(i))) return rc;
+    g.cur = g.M + 1;
+    g.dir = -1;
+    return 0;
+  }
+
+  // add_slice_sequence_left (stack.jl:278-293): element idx+1 from element idx (0-based)
+  int add_left(int idx) {
+    int rc;
+    if ((rc = copy(NM(M_CURU), US(idx)))) return rc;
+    for (int sl = idx * g.s; sl < (idx + 1) * g.s; ++sl)
+      if ((rc = applyB(NM(M_CURU), B_LEFT, sl))) return rc;
+    LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_CURU), DS(idx)));
+    if ((rc = udt(NM(M_CURU), DS(idx + 1), US(idx + 1), NM(M_QRT)))) return rc;
+    return gemm(TS(idx + 1), NM(M_QRT), OP_N, TS(idx), OP_N);
+  }
+
+  // add_slice_sequence_right (stack.jl:299-313): element idx from element idx+1
+  int add_right(int idx) {
+    int rc;
+    if ((rc = copy(NM(M_CURU), US(idx + 1)))) return rc;
+    for (int sl = (idx + 1) * g.s - 1; sl >= idx * g.s; --sl)
+      if ((rc = applyB(NM(M_CURU), B_DAG_LEFT, sl))) return rc;
+    LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_CURU), DS(idx + 1)));
+    if ((rc = udt(NM(M_CURU), DS(idx), US(idx), NM(M_QRT)))) return rc;
+    return gemm(TS(idx), NM(M_QRT), OP_N, TS(idx + 1), OP_N);
+  }
+
+  int load_left(int k) {
+    int rc;
+    if ((rc = copy(NM(M_UL), US(k)))) return rc;
+    if ((rc = copy(NM(M_TL), TS(k)))) return rc;
+    return copyv(NV(V_DL), DS(k));
+  }
+  int load_right(int k) {
+    int rc;
+    if ((rc = copy(NM(M_UR), US(k)))) return rc;
+    if ((rc = copy(NM(M_TR), TS(k)))) return rc;
+    return copyv(NV(V_DR), DS(k));
+  }
+
+  // calculate_greens (stack.jl:338-369) [+ calculate_logdet (stack.jl:377-385)]
+  int calculate_greens(bool logdet) {
+    int rc;
+    // tmp = diag(Dl) Tl Tr^H diag(Dr)
+    if ((rc = gemm(NM(M_TMP), NM(M_TL), OP_N, NM(M_TR), OP_C, NV(V_DL), NV(V_DR)))) return rc;
+    // U1, D1, T1 = udt(tmp)
+    if ((rc = udt(NM(M_TMP), NV(V_D1), NM(M_QRQ), NM(M_QRT)))) return rc;
+    // s.U = Ul U1  -> TMP2
+    if ((rc = gemm(NM(M_TMP2), NM(M_UL), OP_N, NM(M_QRQ), OP_N))) return rc;
+    // tmp2 = T1 Ur^H -> TMP3
+    if ((rc = gemm(NM(M_TMP3), NM(M_QRT), OP_N, NM(M_UR), OP_C))) return rc;
+    // tmp = s.U^H / tmp2 (myrdiv, linalg.jl:61) = s.U^H inv(tmp2)
+    if ((rc = copy(NM(M_TMP4), NM(M_TMP3)))) return rc;
+    if ((rc = inverse(NM(M_TMP4)))) return rc;
+    if ((rc = gemm(NM(M_TMP), NM(M_TMP2), OP_C, NM(M_TMP4), OP_N))) return rc;
+    // tmp += diag(D1); u, d, t = udt(tmp)
+    LCHK(launch_add_diag(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D1)));
+    if ((rc = udt(NM(M_TMP), NV(V_D2), NM(M_QRQ), NM(M_QRT)))) return rc;
+    // s.T = inv(t tmp2)
+    if ((rc = gemm(NM(M_TG), NM(M_QRT), OP_N, NM(M_TMP3), OP_N))) return rc;
+    if ((rc = inverse(NM(M_TG)))) return rc;
+    if (logdet) LCHK(launch_sum_log(s(), d(), nc(), g.n, NV(V_D2), S_LOGDET, 0));  // log d (before 1/d)
+    // s.U = (s.U u)^H
+    if ((rc = gemm(NM(M_TMP), NM(M_TMP2), OP_N, NM(M_QRQ), OP_N))) return rc;
+    LCHK(launch_adjoint(s(), d(), nc(), g.n, NM(M_UG), NM(M_TMP)));
+    // s.d = 1 ./ s.d
+    LCHK(launch_recip_vec(s(), d(), nc(), g.n, NV(V_DG), NV(V_D2)));
+    // greens = s.T diag(s.d) s.U
+    LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, NM(M_TMP3), NM(M_UG), NV(V_DG)));
+    if ((rc = gemm(NM(M_G), NM(M_TG), OP_N, NM(M_TMP3), OP_N))) return rc;
+    if (logdet) {
+      // log_det = log|det U| + sum log(1/d) + log|det T|, with log|det T| = -log|det(t tmp2)|
+      // (the inverse above left log|det(t tmp2)| in S_LOGDET_A) and sum log(1/d) = -sum log d
+      LCHK(launch_add_scalar(s(), d(), nc(), S_LOGDET, S_LOGDET_A));  // S = log d + log|det(t tmp2)|
+      LCHK(launch_negate_scalar(s(), d(), nc(), S_LOGDET));
+      if ((rc = copy(NM(M_TMP4), NM(M_UG)))) return rc;
+      if ((rc = inverse(NM(M_TMP4)))) return rc;  // log|det U| into S_LOGDET_A
+      LCHK(launch_add_scalar(s(), d(), nc(), S_LOGDET, S_LOGDET_A));
+    }
+    return 0;
+  }
+
+  // propagate (stack.jl:391-499)
+  int propagate() {
+    int rc;
+    const int M = g.M, sm = g.s;
+    g.recomputed = false;
+    if (g.dir == 1) {
+      if (g.cur % sm == 0) {
+        g.cur += 1;
+        if (g.cur == 1) {
+          if ((rc = load_right(0))) return rc;
+          if ((rc = identity_elem(0))) return rc;
+          if ((rc = load_left(0))) return rc;
+          return calculate_greens(true);
+        } else if (g.cur <= M) {
+          const int idx = (g.cur - 1) / sm;
+          if ((rc = load_right(idx))) return rc;
+          if ((rc = add_left(idx - 1))) return rc;
+          if ((rc = load_left(idx))) return rc;
+          if (g.all_checks) {
+            if ((rc = copy(NM(M_GW), NM(M_G)))) return rc;
+            if ((rc = wrap(NM(M_GW), g.cur - 1, 1))) return rc;
+          }
+          if ((rc = calculate_greens(false))) return rc;
+          if (g.all_checks) {
+            LCHK(launch_maxdiff(s(), d(), nc(), g.n, NM(M_GW), NM(M_G)));
+            g.recomputed = true;
+          }
+          return 0;
+        } else {
+          if ((rc = add_left(g.nelem - 2))) return rc;
+          g.dir = -1;
+          g.cur = M + 1;
+          return propagate();
+        }
+      }
+      if ((rc = wrap(NM(M_G), g.cur, 1))) return rc;
+      g.cur += 1;
+      return 0;
+    }
+    if ((g.cur - 1) % sm == 0) {
+      g.cur -= 1;
+      if (g.cur == M) {
+        const int last = g.nelem - 1;
+        if ((rc = load_left(last))) return rc;
+        if ((rc = identity_elem(last))) return rc;
+        if ((rc = load_right(last))) return rc;
+        if ((rc = calculate_greens(true))) return rc;
+        return wrap(NM(M_G), g.cur + 1, -1);
+      } else if (g.cur > 0) {
+        const int idx = g.cur / sm + 1;
+        if ((rc = load_left(idx - 1))) return rc;
+        if ((rc = add_right(idx - 1))) return rc;
+        if ((rc = load_right(idx - 1))) return rc;
+        if (g.all_checks)
+          if ((rc = copy(NM(M_GW), NM(M_G)))) return rc;
+        if ((rc = calculate_greens(false))) return rc;
+        if (g.all_checks) {
+          LCHK(launch_maxdiff(s(), d(), nc(), g.n, NM(M_GW), NM(M_G)));
+          g.recomputed = true;
+        }
+        return wrap(NM(M_G), g.cur + 1, -1);
+      } else {
+        if ((rc = add_right(0))) return rc;
+        g.dir = 1;
+        g.cur = 0;
+        return propagate();
+      }
+    }
+    if ((rc = wrap(NM(M_G), g.cur, -1))) return rc;
+    g.cur -= 1;
+    return 0;
+  }
+
+  int local_updates(double box, int dolog) {
+    if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "local_updates outside slices 1..M");
+    LCHK(launch_local_updates(s(), d(), nc(), g.n, g.cur - 1, nullptr, box, g.wg_local, dolog));
+    return 0;
+  }
+};
+
+int check_device_error(dqmc_batch* b) {
+  for (auto& cm : b->chains) {
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, cm.d.error, sizeof(int), hipMemcpyDeviceToHost));
+    if (e == 1) return fail(DQMC_E_DEVICE, "device: chain-group barrier timed out");
+    if (e == 2) return fail(DQMC_E_DEVICE, "device: uniform stream exhausted (upload more with set_uniforms)");
+  }
+  return 0;
+}
+
+int batch_sync(dqmc_batch* b) {
+  for (auto& g : b->groups) HIPCHK(hipStreamSynchronize(g.stream));
+  return check_device_error(b);
+}
+
+int batch_create(const dqmc_params* p, const dqmc_operators* ops, int nchains, int device, dqmc_batch** out) {
+  if (!out || nchains < 1 || !p || !ops) return fail(DQMC_E_ARG, "bad batch arguments");
+  for (int i = 0; i < nchains; ++i) {
+    int rc = validate(&p[i], &ops[i]);
+    if (rc) return rc;
+  }
+  HIPCHK(hipSetDevice(device));
+  auto* b = new dqmc_batch();
+  b->device = device;
+  b->chains.resize(nchains);
+  for (int i = 0; i < nchains; ++i) {
+    int rc = chain_create(b->chains[i], &p[i], &ops[i]);
+    if (rc) {
+      for (auto& cm : b->chains) chain_free(cm);
+      delete b;
+      return rc;
+    }
+  }
+  // shape groups: identical (L, M, safe_mult, checkerboard, all_checks)
+  std::map<std::tuple<int, int, int, int, int>, int> idx;
+  b->chain_group.resize(nchains);
+  for (int i = 0; i < nchains; ++i) {
+    const dqmc_params& q = p[i];
+    auto key = std::make_tuple(q.L, q.slices, q.safe_mult, q.checkerboard, q.all_checks);
+    auto it = idx.find(key);
+    int gi;
+    if (it == idx.end()) {
+      gi = (int)b->groups.size();
+      idx[key] = gi;
+      Group g;
+      g.L = q.L;
+      g.N = q.L * q.L;
+      g.n = 4 * g.N;
+      g.M = q.slices;
+      g.s = q.safe_mult;
+      g.nelem = q.slices / q.safe_mult + 1;
+      g.chkr = q.checkerboard;
+      g.all_checks = q.all_checks != 0;
+      b->groups.push_back(g);
+    } else {
+      gi = it->second;
+    }
+    b->groups[gi].chains.push_back(i);
+    b->chain_group[i] = gi;
+  }
+  for (auto& g : b->groups) {
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    default_groups(g);
+    int rc = upload_desc(b, g);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMalloc(&b->d_scratch, sizeof(double) * 128));
+  *out = b;
+  return 0;
+}
+
+int batch_destroy(dqmc_batch* b) {
+  if (!b) return 0;
+  (void)hipDeviceSynchronize();
+  for (auto& g : b->groups) {
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    if (g.d_desc) (void)hipFree(g.d_desc);
+  }
+  for (auto& cm : b->chains) chain_free(cm);
+  if (b->d_scratch) (void)hipFree(b->d_scratch);
+  delete b;
+  return 0;
+}
+
+int set_uniforms(dqmc_batch* b, int chain, const double* u, long long count) {
+  if (chain < 0 || chain >= (int)b->chains.size() || !u || count < 0) return fail(DQMC_E_ARG, "bad uniforms");
+  ChainMem& cm = b->chains[chain];
+  Group& g = b->groups[b->chain_group[chain]];
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (count > cm.ucap) {
+    if (cm.uniforms) HIPCHK(hipFree(cm.uniforms));
+    HIPCHK(hipMalloc(&cm.uniforms, sizeof(double) * std::max<long long>(count, 1)));
+    cm.ucap = count;
+  }
+  HIPCHK(hipMemcpy(cm.uniforms, u, sizeof(double) * count, hipMemcpyHostToDevice));
+  long long zero = 0;
+  HIPCHK(hipMemcpy(cm.d.ucursor, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  cm.d.uniforms = cm.uniforms;
+  cm.d.n_uniforms = count;
+  return upload_desc(b, g);
+}
+
+int get_stats(dqmc_batch* b, int chain, dqmc_stats* st) {
+  if (!st || chain < 0 || chain >= (int)b->chains.size()) return fail(DQMC_E_ARG, "bad stats args");
+  int rc = batch_sync(b);
+  ChainMem& cm = b->chains[chain];
+  Group& g = b->groups[b->chain_group[chain]];
+  double sc[kNumScalar];
+  HIPCHK(hipMemcpy(sc, cm.d.scal, sizeof(sc), hipMemcpyDeviceToHost));
+  long long cur = 0;
+  HIPCHK(hipMemcpy(&cur, cm.d.ucursor, sizeof(cur), hipMemcpyDeviceToHost));
+  int err = 0;
+  HIPCHK(hipMemcpy(&err, cm.d.error, sizeof(err), hipMemcpyDeviceToHost));
+  st->current_slice = g.cur;
+  st->direction = g.dir;
+  st->boson_action = sc[S_BOSON_ACTION];
+  st->log_det = sc[S_LOGDET];
+  st->drift_last = sc[S_DRIFT_LAST];
+  st->drift_max = sc[S_DRIFT_MAX];
+  st->acc_last = sc[S_ACC_SLICE] / cm.N;
+  st->accepted = sc[S_ACC_TOTAL];
+  st->proposed = sc[S_PROP_TOTAL];
+  st->nonreal = sc[S_NONREAL];
+  st->min_ratio_gap = sc[S_MINRATIO];
+  st->uniforms_used = cur;
+  st->device_error = err;
+  (void)rc;
+  return 0;
+}
+
+int group_init(dqmc_batch*, Group& g) {
+  Ops o(g);
+  int rc;
+  LCHK(launch_boson_action(g.stream, g.d_desc, g.nch(), g.N));
+  LCHK(launch_ones_vec(g.stream, g.d_desc, g.nch(), g.n, NV(V_ONES)));
+  if ((rc = o.build_stack())) return rc;
+  return o.propagate();
+}
+
+int group_sweeps(Group& g, int nsweeps, double box) {
+  Ops o(g);
+  int rc;
+  for (int sw = 0; sw < nsweeps; ++sw)
+    for (int step = 0; step < g.M; ++step) {
+      if ((rc = o.propagate())) return rc;
+      if ((rc = o.local_updates(box, 0))) return rc;
+    }
+  return 0;
+}
+
+// host <-> device matrix helpers for the unit surfaces of a single-chain context
+int put_mat(dqmc_batch* b, MatRef r, const double* host) {
+  ChainMem& cm = b->chains[0];
+  const size_t nn = (size_t)cm.n * cm.n;
+  cplx* dst = r.kind == MK_NAMED ? cm.d.mat[r.idx] : (r.kind == MK_USTACK ? cm.d.ustack : cm.d.tstack) + nn * r.idx;
+  HIPCHK(hipMemcpyAsync(dst, host, nn * sizeof(cplx), hipMemcpyHostToDevice, b->groups[0].stream));
+  return 0;
+}
+int get_mat(dqmc_batch* b, MatRef r, double* host) {
+  ChainMem& cm = b->chains[0];
+  const size_t nn = (size_t)cm.n * cm.n;
+  cplx* src = r.kind == MK_NAMED ? cm.d.mat[r.idx] : (r.kind == MK_USTACK ? cm.d.ustack : cm.d.tstack) + nn * r.idx;
+  HIPCHK(hipMemcpyAsync(host, src, nn * sizeof(cplx), hipMemcpyDeviceToHost, b->groups[0].stream));
+  HIPCHK(hipStreamSynchronize(b->groups[0].stream));
+  return 0;
+}
+int put_vec(dqmc_batch* b, VecRef r, const double* host) {
+  ChainMem& cm = b->chains[0];
+  double* dst = r.kind == VK_NAMED ? cm.d.vec[r.idx] : cm.d.dstack + (size_t)cm.n * r.idx;
+  HIPCHK(hipMemcpyAsync(dst, host, cm.n * sizeof(double), hipMemcpyHostToDevice, b->groups[0].stream));
+  return 0;
+}
+int get_vec(dqmc_batch* b, VecRef r, double* host) {
+  ChainMem& cm = b->chains[0];
+  double* src = r.kind == VK_NAMED ? cm.d.vec[r.idx] : cm.d.dstack + (size_t)cm.n * r.idx;
+  HIPCHK(hipMemcpyAsync(host, src, cm.n * sizeof(double), hipMemcpyDeviceToHost, b->groups[0].stream));
+  HIPCHK(hipStreamSynchronize(b->groups[0].stream));
+  return 0;
+}
+
+double read_scalar(dqmc_batch* b, int chain, int slot) {
+  double v = 0;
+  (void)hipMemcpy(&v, b->chains[chain].d.scal + slot, sizeof(double), hipMemcpyDeviceToHost);
+  return v;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+#define CTX_CHECK(ctx) \
+  if (!(ctx) || !(ctx)->b) return fail(DQMC_E_ARG, "null context")
+#define RC(x)              \
+  do {                     \
+    int _rc = (x);         \
+    if (_rc) return _rc;   \
+  } while (0)
+
+extern "C" {
+
+const char* dqmc_last_error(void) { return g_err.c_str(); }
+
+int dqmc_ctx_create(const dqmc_params* p, const dqmc_operators* ops, int device, dqmc_ctx** out) {
+  if (!out) return fail(DQMC_E_ARG, "null out");
+  dqmc_batch* b = nullptr;
+  RC(batch_create(p, ops, 1, device, &b));
+  *out = new dqmc_ctx{b};
+  return 0;
+}
+
+int dqmc_ctx_destroy(dqmc_ctx* ctx) {
+  if (!ctx) return 0;
+  batch_destroy(ctx->b);
+  delete ctx;
+  return 0;
+}
+
+int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  if (wg_qr > kMaxWG || wg_inv > kMaxWG || wg_local > kMaxWG) return fail(DQMC_E_ARG, "too many workgroups");
+  if (wg_qr > 0) g.wg_qr = wg_qr;
+  if (wg_inv > 0) g.wg_inv = std::max(wg_inv, (g.n + 511) / 512);
+  if (wg_local > 0) g.wg_local = wg_local;
+  return 0;
+}
+
+int dqmc_set_field(dqmc_ctx* ctx, const double* phi) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_set_field(ctx->b, 0, phi);
+}
+int dqmc_get_field(dqmc_ctx* ctx, double* phi) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_get_field(ctx->b, 0, phi);
+}
+int dqmc_set_uniforms(dqmc_ctx* ctx, const double* u, long long count) {
+  CTX_CHECK(ctx);
+  return set_uniforms(ctx->b, 0, u, count);
+}
+
+int dqmc_init(dqmc_ctx* ctx) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_init(ctx->b);
+}
+
+int dqmc_build_stack(dqmc_ctx* ctx) {
+  CTX_CHECK(ctx);
+  Ops o(ctx->b->groups[0]);
+  return o.build_stack();
+}
+
+int dqmc_propagate(dqmc_ctx* ctx, int* current_slice, int* direction, double* drift) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  Ops o(g);
+  RC(o.propagate());
+  if (current_slice) *current_slice = g.cur;
+  if (direction) *direction = g.dir;
+  if (drift) {
+    RC(batch_sync(ctx->b));
+    *drift = g.recomputed ? read_scalar(ctx->b, 0, S_DRIFT_LAST) : -1.0;
+  }
+  return 0;
+}
+
+int dqmc_local_updates(dqmc_ctx* ctx, double box, double* acc_rate) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  Ops o(g);
+  const int dolog = ctx->b->chains[0].log_cap > 0;
+  RC(o.local_updates(box, dolog));
+  if (acc_rate) {
+    RC(batch_sync(ctx->b));
+    *acc_rate = read_scalar(ctx->b, 0, S_ACC_SLICE) / ctx->b->chains[0].N;
+  }
+  return 0;
+}
+
+int dqmc_sweep(dqmc_ctx* ctx, int nsweeps, double box) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_sweep(ctx->b, nsweeps, &box);
+}
+
+int dqmc_synchronize(dqmc_ctx* ctx) {
+  CTX_CHECK(ctx);
+  return batch_sync(ctx->b);
+}
+
+int dqmc_get_greens(dqmc_ctx* ctx, double* G) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_get_greens(ctx->b, 0, G);
+}
+
+int dqmc_set_greens(dqmc_ctx* ctx, const double* G) {
+  CTX_CHECK(ctx);
+  if (!G) return fail(DQMC_E_ARG, "null G");
+  RC(put_mat(ctx->b, NM(M_G), G));
+  return batch_sync(ctx->b);
+}
+
+int dqmc_get_stack(dqmc_ctx* ctx, int k, double* U, double* D, double* T) {
+  CTX_CHECK(ctx);
+  ChainMem& cm = ctx->b->chains[0];
+  if (k < 0 || k >= cm.nelem) return fail(DQMC_E_ARG, "stack index out of range");
+  RC(batch_sync(ctx->b));
+  if (U) RC(get_mat(ctx->b, US(k), U));
+  if (T) RC(get_mat(ctx->b, TS(k), T));
+  if (D) RC(get_vec(ctx->b, DS(k), D));
+  return 0;
+}
+
+int dqmc_get_stats(dqmc_ctx* ctx, dqmc_stats* st) {
+  CTX_CHECK(ctx);
+  return get_stats(ctx->b, 0, st);
+}
+
+int dqmc_reset_stats(dqmc_ctx* ctx) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  LCHK(launch_reset_stats(g.stream, g.d_desc, g.nch()));
+  return batch_sync(ctx->b);
+}
+
+int dqmc_set_proposal_log(dqmc_ctx* ctx, long long capacity) {
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  RC(batch_sync(b));
+  ChainMem& cm = b->chains[0];
+  if (cm.log_buf) HIPCHK(hipFree(cm.log_buf));
+  cm.log_buf = nullptr;
+  cm.log_cap = 0;
+  if (capacity > 0) {
+    HIPCHK(hipMalloc(&cm.log_buf, sizeof(double) * 4 * capacity));
+    cm.log_cap = capacity;
+  }
+  cm.d.prop_log = cm.log_buf;
+  cm.d.log_cap = cm.log_cap;
+  long long zero = 0;
+  HIPCHK(hipMemcpy(cm.d.log_count, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  return upload_desc(b, b->groups[0]);
+}
+
+int dqmc_get_proposal_log(dqmc_ctx* ctx, double* out, long long* count) {
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  RC(batch_sync(b));
+  ChainMem& cm = b->chains[0];
+  long long c = 0;
+  HIPCHK(hipMemcpy(&c, cm.d.log_count, sizeof(c), hipMemcpyDeviceToHost));
+  if (count) *count = c;
+  const long long m = std::min(c, cm.log_cap);
+  if (out && m > 0) HIPCHK(hipMemcpy(out, cm.log_buf, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dqmc_apply_B(dqmc_ctx* ctx, int slice, int which, double* X) {
+  CTX_CHECK(ctx);
+  ChainMem& cm = ctx->b->chains[0];
+  if (!X || slice < 0 || slice >= cm.M || which < 0 || which > 4) return fail(DQMC_E_ARG, "bad apply_B args");
+  Ops o(ctx->b->groups[0]);
+  RC(put_mat(ctx->b, NM(M_GRAD), X));
+  RC(o.applyB(NM(M_GRAD), (BOp)which, slice));
+  RC(get_mat(ctx->b, NM(M_GRAD), X));
+  return check_device_error(ctx->b);
+}
+
+int dqmc_wrap_greens(dqmc_ctx* ctx, double* G, int slice, int direction) {
+  CTX_CHECK(ctx);
+  ChainMem& cm = ctx->b->chains[0];
+  if (!G || (direction != 1 && direction != -1)) return fail(DQMC_E_ARG, "bad wrap args");
+  const int sl0 = direction == 1 ? slice - 1 : slice - 2;
+  if (sl0 < 0 || sl0 >= cm.M) return fail(DQMC_E_ARG, "slice out of range");
+  Ops o(ctx->b->groups[0]);
+  RC(put_mat(ctx->b, NM(M_GRAD), G));
+  RC(o.wrap(NM(M_GRAD), slice, direction));
+  return get_mat(ctx->b, NM(M_GRAD), G);
+}
+
+int dqmc_udt(dqmc_ctx* ctx, const double* A, double* U, double* D, double* T) {
+  CTX_CHECK(ctx);
+  if (!A) return fail(DQMC_E_ARG, "null A");
+  Ops o(ctx->b->groups[0]);
+  RC(put_mat(ctx->b, NM(M_QRA), A));
+  RC(o.udt(NM(M_QRA), NV(V_QRD), NM(M_QRQ), NM(M_QRT)));
+  RC(batch_sync(ctx->b));
+  if (U) RC(get_mat(ctx->b, NM(M_QRQ), U));
+  if (T) RC(get_mat(ctx->b, NM(M_QRT), T));
+  if (D) RC(get_vec(ctx->b, NV(V_QRD), D));
+  return 0;
+}
+
+int dqmc_calculate_greens(dqmc_ctx* ctx, const double* Ul, const double* Dl, const double* Tl, const double* Ur,
+                          const double* Dr, const double* Tr, double* G, double* logdet) {
+  CTX_CHECK(ctx);
+  if (!Ul || !Dl || !Tl || !Ur || !Dr || !Tr) return fail(DQMC_E_ARG, "null UDT factor");
+  Ops o(ctx->b->groups[0]);
+  RC(put_mat(ctx->b, NM(M_UL), Ul));
+  RC(put_mat(ctx->b, NM(M_TL), Tl));
+  RC(put_mat(ctx->b, NM(M_UR), Ur));
+  RC(put_mat(ctx->b, NM(M_TR), Tr));
+  RC(put_vec(ctx->b, NV(V_DL), Dl));
+  RC(put_vec(ctx->b, NV(V_DR), Dr));
+  RC(o.calculate_greens(true));
+  RC(batch_sync(ctx->b));
+  if (G) RC(get_mat(ctx->b, NM(M_G), G));
+  if (logdet) *logdet = read_scalar(ctx->b, 0, S_LOGDET);
+  return 0;
+}
+
+int dqmc_greens_at(dqmc_ctx* ctx, int slice, double* G, double* logdet) {
+  // calc_greens_and_logdet (fermion_measurements.jl:1061-1101) via calc_Bchain(_dagger)
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  if (slice < 1 || slice > g.M) return fail(DQMC_E_ARG, "slice out of range");
+  Ops o(g);
+  const int sm = g.s;
+  // Ur, Dr, Tr = B(slice)' ... B(M)'  (calc_Bchain_dagger, decompose at k % sm == 0 and k == start)
+  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UR), NOVEC));
+  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TR), NOVEC));
+  LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DR)));
+  for (int k = g.M; k >= slice; --k) {
+    RC(o.applyB(NM(M_UR), B_DAG_LEFT, k - 1));
+    if (k % sm == 0 || k == slice) {
+      LCHK(launch_scale_cols(g.stream, g.d_desc, 1, g.n, NM(M_UR), NV(V_DR)));
+      RC(o.udt(NM(M_UR), NV(V_DR), NM(M_QRQ), NM(M_QRT)));
+      RC(o.copy(NM(M_UR), NM(M_QRQ)));
+      RC(o.gemm(NM(M_TMP), NM(M_QRT), OP_N, NM(M_TR), OP_N));
+      RC(o.copy(NM(M_TR), NM(M_TMP)));
+    }
+  }
+  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UL), NOVEC));
+  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TL), NOVEC));
+  LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DL)));
+  for (int k = 1; k <= slice - 1; ++k) {
+    RC(o.applyB(NM(M_UL), B_LEFT, k - 1));
+    if (k % sm == 0 || k == slice - 1) {
+      LCHK(launch_scale_cols(g.stream, g.d_desc, 1, g.n, NM(M_UL), NV(V_DL)));
+      RC(o.udt(NM(M_UL), NV(V_DL), NM(M_QRQ), NM(M_QRT)));
+      RC(o.copy(NM(M_UL), NM(M_QRQ)));
+      RC(o.gemm(NM(M_TMP), NM(M_QRT), OP_N, NM(M_TL), OP_N));
+      RC(o.copy(NM(M_TL), NM(M_TMP)));
+    }
+  }
+  // keep the chain's G: compute into G then restore
+  RC(o.copy(NM(M_GW), NM(M_G)));
+  RC(o.calculate_greens(true));
+  RC(batch_sync(ctx->b));
+  if (G) RC(get_mat(ctx->b, NM(M_G), G));
+  if (logdet) *logdet = read_scalar(ctx->b, 0, S_LOGDET);
+  RC(o.copy(NM(M_G), NM(M_GW)));
+  return batch_sync(ctx->b);
+}
+
+int dqmc_detratio(dqmc_ctx* ctx, int site, const double* new_op, double* det, double* delta, double* Mout) {
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  Group& g = b->groups[0];
+  if (site < 0 || site >= g.N || !new_op) return fail(DQMC_E_ARG, "bad detratio args");
+  if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "detratio outside slices 1..M");
+  HIPCHK(hipMemcpyAsync(b->d_scratch + 100, new_op, 3 * sizeof(double), hipMemcpyHostToDevice, g.stream));
+  LCHK(launch_detratio(g.stream, g.d_desc, g.cur - 1, site, b->d_scratch + 100, b->d_scratch));
+  double out[66];
+  HIPCHK(hipMemcpyAsync(out, b->d_scratch, sizeof(out), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (det) memcpy(det, out, 2 * sizeof(double));
+  if (delta) memcpy(delta, out + 2, 32 * sizeof(double));
+  if (Mout) memcpy(Mout, out + 34, 32 * sizeof(double));
+  return 0;
+}
+
+int dqmc_update_greens(dqmc_ctx* ctx, int site) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  if (site < 0 || site >= g.N) return fail(DQMC_E_ARG, "bad site");
+  LCHK(launch_force_update(g.stream, g.d_desc, g.n, site));
+  return batch_sync(ctx->b);
+}
+
+int dqmc_gemm(dqmc_ctx* ctx, int opA, int opB, const double* A, const double* B, double* C) {
+  CTX_CHECK(ctx);
+  if (!A || !B || !C || opA < 0 || opA > 1 || opB < 0 || opB > 1) return fail(DQMC_E_ARG, "bad gemm args");
+  Ops o(ctx->b->groups[0]);
+  RC(put_mat(ctx->b, NM(M_TMP), A));
+  RC(put_mat(ctx->b, NM(M_TMP2), B));
+  RC(o.gemm(NM(M_TMP3), NM(M_TMP), (GemmOp)opA, NM(M_TMP2), (GemmOp)opB));
+  return get_mat(ctx->b, NM(M_TMP3), C);
+}
+
+int dqmc_inverse(dqmc_ctx* ctx, const double* A, double* Ainv, double* logabsdet) {
+  CTX_CHECK(ctx);
+  if (!A || !Ainv) return fail(DQMC_E_ARG, "bad inverse args");
+  Ops o(ctx->b->groups[0]);
+  RC(put_mat(ctx->b, NM(M_TMP4), A));
+  RC(o.inverse(NM(M_TMP4)));
+  RC(get_mat(ctx->b, NM(M_TMP4), Ainv));
+  RC(check_device_error(ctx->b));
+  if (logabsdet) *logabsdet = read_scalar(ctx->b, 0, S_LOGDET_A);
+  return 0;
+}
+
+int dqmc_boson_action(dqmc_ctx* ctx, double* S) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  // computed into a scratch copy of the scalar so the running p.boson_action is kept
+  double keep = read_scalar(ctx->b, 0, S_BOSON_ACTION);
+  LCHK(launch_boson_action(g.stream, g.d_desc, 1, g.N));
+  RC(batch_sync(ctx->b));
+  if (S) *S = read_scalar(ctx->b, 0, S_BOSON_ACTION);
+  HIPCHK(hipMemcpy(ctx->b->chains[0].d.scal + S_BOSON_ACTION, &keep, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int dqmc_grad(dqmc_ctx* ctx, double* grad, double* S) {
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  Group& g = b->groups[0];
+  ChainMem& cm = b->chains[0];
+  Ops o(g);
+  double* dgrad = nullptr;
+  double** dptr = nullptr;
+  HIPCHK(hipMalloc(&dgrad, sizeof(double) * 3 * cm.N * cm.M));
+  HIPCHK(hipMalloc(&dptr, sizeof(double*)));
+  HIPCHK(hipMemcpy(dptr, &dgrad, sizeof(double*), hipMemcpyHostToDevice));
+  int rc = 0;
+  do {
+    if ((rc = o.build_stack())) break;
+    if ((rc = o.propagate())) break;  // -> slice M, direction -1, with log det
+    LCHK(launch_boson_grad(g.stream, g.d_desc, 1, cm.N * cm.M, dptr));
+    LCHK(launch_boson_action(g.stream, g.d_desc, 1, g.N));
+    // G(l) for l = M .. 1 on the down pass
+    for (int step = 0; step < g.M; ++step) {
+      LCHK(launch_grad_slice(g.stream, g.d_desc, 1, g.N, g.cur - 1, dptr));
+      if (step + 1 < g.M)
+        if ((rc = o.propagate())) break;
+    }
+    if (rc) break;
+    if ((rc = batch_sync(b))) break;
+    if (grad) HIPCHK(hipMemcpy(grad, dgrad, sizeof(double) * 3 * cm.N * cm.M, hipMemcpyDeviceToHost));
+    if (S) *S = read_scalar(b, 0, S_BOSON_ACTION) + read_scalar(b, 0, S_LOGDET);
+  } while (0);
+  (void)hipFree(dgrad);
+  (void)hipFree(dptr);
+  return rc;
+}
+
+// ---- batches ---------------------------------------------------------------
+int dqmc_batch_create(const dqmc_params* p, const dqmc_operators* ops, int nchains, int device, dqmc_batch** out) {
+  return batch_create(p, ops, nchains, device, out);
+}
+int dqmc_batch_destroy(dqmc_batch* b) { return batch_destroy(b); }
+
+int dqmc_batch_set_field(dqmc_batch* b, int chain, const double* phi) {
+  if (!b || !phi || chain < 0 || chain >= (int)b->chains.size()) return fail(DQMC_E_ARG, "bad set_field");
+  ChainMem& cm = b->chains[chain];
+  RC(batch_sync(b));
+  HIPCHK(hipMemcpy(cm.d.phi, phi, sizeof(double) * 3 * cm.N * cm.M, hipMemcpyHostToDevice));
+  return 0;
+}
+int dqmc_batch_get_field(dqmc_batch* b, int chain, double* phi) {
+  if (!b || !phi || chain < 0 || chain >= (int)b->chains.size()) return fail(DQMC_E_ARG, "bad get_field");
+  ChainMem& cm = b->chains[chain];
+  RC(batch_sync(b));
+  HIPCHK(hipMemcpy(phi, cm.d.phi, sizeof(double) * 3 * cm.N * cm.M, hipMemcpyDeviceToHost));
+  return 0;
+}
+int dqmc_batch_set_uniforms(dqmc_batch* b, int chain, const double* u, long long count) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  return set_uniforms(b, chain, u, count);
+}
+int dqmc_batch_init(dqmc_batch* b) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  for (auto& g : b->groups) RC(group_init(b, g));
+  return 0;
+}
+int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box) {
+  if (!b || !box || nsweeps < 0) return fail(DQMC_E_ARG, "bad sweep args");
+  // groups are independent streams: enqueue them step-interleaved so their kernels overlap
+  for (auto& g : b->groups) {
+    const double bx = box[g.chains[0]];
+    for (int ci : g.chains)
+      if (box[ci] != bx) return fail(DQMC_E_ARG, "chains of one shape group must share the box size");
+  }
+  std::vector<int> steps(b->groups.size(), 0);
+  bool more = true;
+  while (more) {
+    more = false;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+      Group& g = b->groups[gi];
+      const int total = nsweeps * g.M;
+      if (steps[gi] >= total) continue;
+      Ops o(g);
+      RC(o.propagate());
+      RC(o.local_updates(box[g.chains[0]], 0));
+      steps[gi] += 1;
+      more = more || steps[gi] < total;
+    }
+  }
+  return 0;
+}
+int dqmc_batch_synchronize(dqmc_batch* b) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  return batch_sync(b);
+}
+int dqmc_batch_get_greens(dqmc_batch* b, int chain, double* G) {
+  if (!b || !G || chain < 0 || chain >= (int)b->chains.size()) return fail(DQMC_E_ARG, "bad get_greens");
+  RC(batch_sync(b));
+  ChainMem& cm = b->chains[chain];
+  HIPCHK(hipMemcpy(G, cm.d.mat[M_G], sizeof(cplx) * cm.n * cm.n, hipMemcpyDeviceToHost));
+  return 0;
+}
+int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  return get_stats(b, chain, st);
+}
+int dqmc_batch_use_graph(dqmc_batch* b, int enable) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  b->use_graph = enable != 0;
+  return 0;
+}
+
+}  // extern "C"

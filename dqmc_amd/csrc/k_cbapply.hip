@@ -1,0 +1,258 @@
+// Slice-matrix application B_l = H_B^(1/2) H_A H_B^(1/2) e^{dtau mu} eV_l and its
+// inverse / adjoint, from the left (on columns) or the right (on rows).
+//
+// Reference: slice_matrices.jl:101-226 (Assaad checkerboard, five sparse passes
+// each followed by a copy), interactions.jl:35-88 (eV rebuilt per call) and
+// slice_matrices.jl:25-94 (dense, no checkerboard). Here every factor is a set
+// of disjoint 4x4 mixes (a site's 4 flavours for eV, a plaquette's 4 sites
+// within one flavour block for the hoppings, a diagonal for mu), so a tile of
+// vectors is loaded into LDS once, all five factors are applied in LDS, and the
+// tile is written back: one HBM read and one write of the operand in total.
+#include "chain.hpp"
+
+namespace dqmc {
+
+enum StageKind : int { ST_EV = 0, ST_MU = 1, ST_MUINV = 2, ST_PLAQ = 3 };
+enum Mode : int { MODE_N = 0, MODE_T = 1, MODE_C = 2 };
+// plaquette variants in ChainDesc::plaq
+enum PlaqVar : int { PV_HALF = 0, PV_HALF_INV = 1, PV_FULL = 2, PV_FULL_INV = 3 };
+
+struct Stage {
+  int kind, mode, var, group;
+  double power;
+};
+struct StageList {
+  Stage st[5];
+  int count;
+};
+
+// y = op(P) x for a 4x4 row-major P
+__device__ inline void mix4(const cplx* P, int mode, cplx x[4], cplx y[4]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    cplx acc = mk(0, 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (mode == MODE_N)
+        cfma(acc, P[a * 4 + k], x[k]);
+      else if (mode == MODE_T)
+        cfma(acc, P[k * 4 + a], x[k]);
+      else
+        cfma_cj(acc, P[k * 4 + a], x[k]);
+    }
+    y[a] = acc;
+  }
+}
+
+// eV site block (interactions.jl:102-140): [[C,S,0,R],[S*,C,-R,0],[0,-R,C,S*],[R,0,S,C]]
+__device__ inline void ev_block(double C, cplx S, double R, cplx E[16]) {
+  const cplx z = mk(0, 0);
+  E[0] = mk(C, 0);   E[1] = S;          E[2] = z;          E[3] = mk(R, 0);
+  E[4] = conj(S);    E[5] = mk(C, 0);   E[6] = mk(-R, 0);  E[7] = z;
+  E[8] = z;          E[9] = mk(-R, 0);  E[10] = mk(C, 0);  E[11] = conj(S);
+  E[12] = mk(R, 0);  E[13] = z;         E[14] = S;         E[15] = mk(C, 0);
+}
+
+// C, S, R per site for power +1 (interactions.jl:52-58); the power sign is applied later.
+__device__ inline void ev_coeffs(const ChainDesc& c, int l, int i, double& C, cplx& S, double& R) {
+  const double* f = c.phi + 3 * ((size_t)i + (size_t)c.N * l);
+  const double nrm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  const double x = c.lam * c.dtau * nrm;
+  const double sh = sinh(x) / nrm;
+  C = cosh(x);
+  S = mk(-f[0] * sh, f[1] * sh);
+  R = -f[2] * sh;
+}
+
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, MatRef Xr, StageList sl, int slice,
+                                                      const int* slices, int vpb) {
+  const ChainDesc& c = chains[blockIdx.y];
+  const int n = c.n, N = c.N;
+  const int v0 = blockIdx.x * vpb;
+  if (v0 >= n) return;
+  const int nv = min(vpb, n - v0);
+  const int l = slices ? slices[blockIdx.y] : slice;
+  cplx* X = matptr(c, Xr);
+
+  extern __shared__ double smem[];
+  cplx* vec = reinterpret_cast<cplx*>(smem);          // [vpb][n]
+  double* evc = smem + 2 * (size_t)vpb * n;           // [N][4] C, S.re, S.im, R
+
+  bool need_ev = false;
+  for (int s = 0; s < sl.count; ++s) need_ev |= (sl.st[s].kind == ST_EV);
+  if (need_ev) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      double C, R;
+      cplx S;
+      ev_coeffs(c, l, i, C, S, R);
+      evc[4 * i + 0] = C;
+      evc[4 * i + 1] = S.re;
+      evc[4 * i + 2] = S.im;
+      evc[4 * i + 3] = R;
+    }
+  }
+  // load
+  if (!ROWS) {
+    for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
+      const int v = idx / n, k = idx - v * n;
+      vec[v * n + k] = X[(size_t)(v0 + v) * n + k];
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
+      const int k = idx / nv, v = idx - k * nv;
+      vec[v * n + k] = X[(size_t)k * n + (v0 + v)];
+    }
+  }
+  __syncthreads();
+
+  const int nplaq = N / 4;
+  for (int s = 0; s < sl.count; ++s) {
+    const Stage st = sl.st[s];
+    if (st.kind == ST_MU || st.kind == ST_MUINV) {
+      const double* mu = (st.kind == ST_MU) ? c.expmu : c.expmu_inv;
+      for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
+        const int v = idx / n, k = idx - v * n;
+        vec[v * n + k] = mu[k] * vec[v * n + k];
+      }
+    } else if (st.kind == ST_EV) {
+      for (int idx = threadIdx.x; idx < nv * N; idx += blockDim.x) {
+        const int v = idx / N, i = idx - v * N;
+        cplx E[16];
+        const double p = st.power;
+        ev_block(evc[4 * i], mk(p * evc[4 * i + 1], p * evc[4 * i + 2]), p * evc[4 * i + 3], E);
+        cplx x[4], y[4];
+        cplx* base = vec + (size_t)v * n + i;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) x[a] = base[a * N];
+        mix4(E, st.mode, x, y);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) base[a * N] = y[a];
+      }
+    } else {  // ST_PLAQ
+      const cplx* P0 = c.plaq + (((size_t)st.var * 2 + st.group) * 4) * nplaq * 16;
+      const int* sites = c.plaq_sites + (size_t)st.group * nplaq * 4;
+      for (int idx = threadIdx.x; idx < nv * 4 * nplaq; idx += blockDim.x) {
+        const int v = idx / (4 * nplaq);
+        const int rem = idx - v * 4 * nplaq;
+        const int b = rem / nplaq, pq = rem - b * nplaq;
+        const cplx* P = P0 + ((size_t)b * nplaq + pq) * 16;
+        const int* st4 = sites + 4 * pq;
+        cplx* base = vec + (size_t)v * n + b * N;
+        cplx x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) x[a] = base[st4[a]];
+        mix4(P, st.mode, x, y);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) base[st4[a]] = y[a];
+      }
+    }
+    __syncthreads();
+  }
+  // store
+  if (!ROWS) {
+    for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
+      const int v = idx / n, k = idx - v * n;
+      X[(size_t)(v0 + v) * n + k] = vec[v * n + k];
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
+      const int k = idx / nv, v = idx - k * nv;
+      X[(size_t)k * n + (v0 + v)] = vec[v * n + k];
+    }
+  }
+}
+
+static Stage mkst(int kind, int mode, int var, int group, double power) {
+  Stage s;
+  s.kind = kind;
+  s.mode = mode;
+  s.var = var;
+  s.group = group;
+  s.power = power;
+  return s;
+}
+
+static hipError_t run_stages(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                             bool rows, const StageList& sl, int slice, const int* slices) {
+  // tile of vectors per block: enough blocks to cover the chip, LDS <= 64 KB
+  int vpb = rows ? 8 : 4;
+  while (vpb > 1 && (size_t)vpb * nmax * 16 > 65536) vpb >>= 1;
+  const size_t lds = (size_t)vpb * nmax * 16 + (size_t)Nmax * 32;
+  dim3 grid((nmax + vpb - 1) / vpb, nchains);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_apply_stages<true>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_apply_stages<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              96 * 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  if (rows)
+    hipLaunchKernelGGL(k_apply_stages<true>, grid, dim3(256), lds, s, chains, X, sl, slice, slices, vpb);
+  else
+    hipLaunchKernelGGL(k_apply_stages<false>, grid, dim3(256), lds, s, chains, X, sl, slice, slices, vpb);
+  return hipGetLastError();
+}
+
+// Checkerboard factor order (slice_matrices.jl:101-226): groups are [A, B] = [0, 1];
+// B = hop_half[B] hop[A] hop_half[B] mu eV.
+hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                               BOp op, int slice, const int* slices) {
+  StageList sl;
+  sl.count = 5;
+  bool rows = false;
+  switch (op) {
+    case B_LEFT:
+      sl.st[0] = mkst(ST_EV, MODE_N, 0, 0, 1.0);
+      sl.st[1] = mkst(ST_MU, MODE_N, 0, 0, 0);
+      sl.st[2] = mkst(ST_PLAQ, MODE_N, PV_HALF, 1, 0);
+      sl.st[3] = mkst(ST_PLAQ, MODE_N, PV_FULL, 0, 0);
+      sl.st[4] = mkst(ST_PLAQ, MODE_N, PV_HALF, 1, 0);
+      break;
+    case B_INV_LEFT:
+      sl.st[0] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 1, 0);
+      sl.st[1] = mkst(ST_PLAQ, MODE_N, PV_FULL_INV, 0, 0);
+      sl.st[2] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 1, 0);
+      sl.st[3] = mkst(ST_MUINV, MODE_N, 0, 0, 0);
+      sl.st[4] = mkst(ST_EV, MODE_N, 0, 0, -1.0);
+      break;
+    case B_DAG_LEFT:
+      sl.st[0] = mkst(ST_PLAQ, MODE_C, PV_HALF, 1, 0);
+      sl.st[1] = mkst(ST_PLAQ, MODE_C, PV_FULL, 0, 0);
+      sl.st[2] = mkst(ST_PLAQ, MODE_C, PV_HALF, 1, 0);
+      sl.st[3] = mkst(ST_MU, MODE_N, 0, 0, 0);
+      sl.st[4] = mkst(ST_EV, MODE_C, 0, 0, 1.0);
+      break;
+    case B_RIGHT:
+      rows = true;
+      sl.st[0] = mkst(ST_PLAQ, MODE_T, PV_HALF, 1, 0);
+      sl.st[1] = mkst(ST_PLAQ, MODE_T, PV_FULL, 0, 0);
+      sl.st[2] = mkst(ST_PLAQ, MODE_T, PV_HALF, 1, 0);
+      sl.st[3] = mkst(ST_MU, MODE_N, 0, 0, 0);
+      sl.st[4] = mkst(ST_EV, MODE_T, 0, 0, 1.0);
+      break;
+    case B_INV_RIGHT:
+      rows = true;
+      sl.st[0] = mkst(ST_EV, MODE_T, 0, 0, -1.0);
+      sl.st[1] = mkst(ST_MUINV, MODE_N, 0, 0, 0);
+      sl.st[2] = mkst(ST_PLAQ, MODE_T, PV_HALF_INV, 1, 0);
+      sl.st[3] = mkst(ST_PLAQ, MODE_T, PV_FULL_INV, 0, 0);
+      sl.st[4] = mkst(ST_PLAQ, MODE_T, PV_HALF_INV, 1, 0);
+      break;
+  }
+  return run_stages(s, chains, nchains, nmax, Nmax, X, rows, sl, slice, slices);
+}
+
+// Only the eV factor (used by the dense no-checkerboard path).
+hipError_t launch_apply_eV(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                           bool rows, int mode, double power, int slice, const int* slices) {
+  StageList sl;
+  sl.count = 1;
+  sl.st[0] = mkst(ST_EV, mode, 0, 0, power);
+  return run_stages(s, chains, nchains, nmax, Nmax, X, rows, sl, slice, slices);
+}
+
+}  // namespace dqmc

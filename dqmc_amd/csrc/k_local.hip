@@ -1,0 +1,448 @@
+// Local updates of one time slice: N sequential Metropolis proposals, each a
+// 4x4 determinant ratio, and on acceptance a rank-4 update of G.
+//
+// Reference: local_updates (local_updates.jl:1-39), calc_detratio (:42-59),
+// update_greens! (:61-95), calc_boson_action_diff (action.jl:57-101),
+// randuniform (dqmc_framework.jl:628-635).
+//
+// One chain group of W workgroups per chain (cooperative launch). Every
+// workgroup evaluates every proposal redundantly from identical inputs (the
+// host uniform stream, an LDS copy of the field slice, the 4x4 block of G), so
+// all reach the same decision without communicating; rejected proposals cost no
+// synchronisation. Workgroup w owns a contiguous block of G's columns. On
+// acceptance each workgroup reads the four site columns of G (the rank-4 left
+// factor) and the four site rows of its own columns, and writes its columns of
+// the updated G into the ping-pong partner buffer, so no workgroup ever writes
+// memory another is reading in the same step: one group barrier per accepted
+// proposal.
+#include "chain.hpp"
+
+namespace dqmc {
+
+__device__ inline void ev_op(double lam_dtau, double o0, double o1, double o2, double power, cplx E[16]) {
+  const double nrm = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+  const double sh = power * sinh(lam_dtau * nrm) / nrm;
+  const double C = cosh(lam_dtau * nrm);
+  const cplx S = mk(-o0 * sh, o1 * sh);
+  const double R = -o2 * sh;
+  const cplx z = mk(0, 0);
+  E[0] = mk(C, 0);   E[1] = S;          E[2] = z;          E[3] = mk(R, 0);
+  E[4] = conj(S);    E[5] = mk(C, 0);   E[6] = mk(-R, 0);  E[7] = z;
+  E[8] = z;          E[9] = mk(-R, 0);  E[10] = mk(C, 0);  E[11] = conj(S);
+  E[12] = mk(R, 0);  E[13] = z;         E[14] = S;         E[15] = mk(C, 0);
+}
+
+// row-major 4x4 product
+__device__ inline void mul4(const cplx* A, const cplx* B, cplx* C) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      cplx acc = mk(0, 0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) cfma(acc, A[i * 4 + k], B[k * 4 + j]);
+      C[i * 4 + j] = acc;
+    }
+}
+
+__device__ inline double cabs1_(cplx a) { return fabs(a.re) + fabs(a.im); }
+
+// det(M) by LU with partial pivoting (first max of |re|+|im|), as LinearAlgebra.det
+__device__ inline cplx det4(const cplx* Min) {
+  cplx a[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) a[e] = Min[e];
+  cplx det = mk(1, 0);
+  for (int k = 0; k < 4; ++k) {
+    int p = k;
+    double bv = cabs1_(a[k * 4 + k]);
+    for (int r = k + 1; r < 4; ++r) {
+      const double v = cabs1_(a[r * 4 + k]);
+      if (v > bv) {
+        bv = v;
+        p = r;
+      }
+    }
+    if (p != k) {
+      for (int j = 0; j < 4; ++j) {
+        const cplx t = a[k * 4 + j];
+        a[k * 4 + j] = a[p * 4 + j];
+        a[p * 4 + j] = t;
+      }
+      det = -det;
+    }
+    const cplx piv = a[k * 4 + k];
+    det = det * piv;
+    const cplx pinv = cinv(piv);
+    for (int r = k + 1; r < 4; ++r) {
+      const cplx l = a[r * 4 + k] * pinv;
+      for (int j = k + 1; j < 4; ++j) a[r * 4 + j] = a[r * 4 + j] - l * a[k * 4 + j];
+    }
+  }
+  return det;
+}
+
+// inverse by Gauss-Jordan with partial pivoting
+__device__ inline void inv4(const cplx* Min, cplx* out) {
+  cplx a[16];
+  int piv[4];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) a[e] = Min[e];
+  for (int k = 0; k < 4; ++k) {
+    int p = k;
+    double bv = cabs1_(a[k * 4 + k]);
+    for (int r = k + 1; r < 4; ++r) {
+      const double v = cabs1_(a[r * 4 + k]);
+      if (v > bv) {
+        bv = v;
+        p = r;
+      }
+    }
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < 4; ++j) {
+        const cplx t = a[k * 4 + j];
+        a[k * 4 + j] = a[p * 4 + j];
+        a[p * 4 + j] = t;
+      }
+    const cplx pinv = cinv(a[k * 4 + k]);
+    a[k * 4 + k] = mk(1, 0);
+    for (int j = 0; j < 4; ++j) a[k * 4 + j] = a[k * 4 + j] * pinv;
+    for (int r = 0; r < 4; ++r) {
+      if (r == k) continue;
+      const cplx f = a[r * 4 + k];
+      a[r * 4 + k] = mk(0, 0);
+      for (int j = 0; j < 4; ++j) a[r * 4 + j] = a[r * 4 + j] - f * a[k * 4 + j];
+    }
+  }
+  for (int k = 3; k >= 0; --k)
+    if (piv[k] != k)
+      for (int r = 0; r < 4; ++r) {
+        const cplx t = a[r * 4 + k];
+        a[r * 4 + k] = a[r * 4 + piv[k]];
+        a[r * 4 + piv[k]] = t;
+      }
+#pragma unroll
+  for (int e = 0; e < 16; ++e) out[e] = a[e];
+}
+
+struct LocalShared {
+  cplx delta[16];
+  cplx minv[16];
+  int accepted;
+  int stop;
+};
+
+__global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, int slice, const int* slices,
+                                                       double box, int dolog) {
+  const ChainDesc& c = chains[blockIdx.y];
+  const int n = c.n, N = c.N, M = c.M;
+  const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+  const int l = slices ? slices[blockIdx.y] : slice;
+  GroupSync g = chain_group(c);
+  // own column block [cb0, cb1)
+  const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
+  const int ncol = cb1 - cb0;
+
+  extern __shared__ double smem[];
+  double* phl = smem;              // [3N] current slice
+  double* phe = phl + 3 * N;       // [3N] earlier slice
+  double* pla = phe + 3 * N;       // [3N] later slice
+  int* nbl = reinterpret_cast<int*>(pla + 3 * N);  // [4N]
+  cplx* Acol = reinterpret_cast<cplx*>(smem + 9 * N + 2 * N + (N & 1));  // [n][4]
+  cplx* Brow = Acol + 4 * (size_t)n;                                      // [4][ncol]
+  __shared__ LocalShared sh;
+
+  const int le = (l == 0) ? M - 1 : l - 1;  // time_neighbors (lattice.jl:144-153)
+  const int la = (l == M - 1) ? 0 : l + 1;
+  for (int e = tid; e < 3 * N; e += blockDim.x) {
+    phl[e] = c.phi[(size_t)3 * N * l + e];
+    phe[e] = c.phi[(size_t)3 * N * le + e];
+    pla[e] = c.phi[(size_t)3 * N * la + e];
+  }
+  for (int e = tid; e < 4 * N; e += blockDim.x) nbl[e] = c.nb[e];
+  __syncthreads();
+
+  const double lam_dtau = c.lam * c.dtau;
+  long long cur = *c.ucursor;
+  long long logi = dolog ? *c.log_count : 0;
+  double dS_acc = 0.0;
+  int nacc = 0, nonreal = 0;
+  double minratio = c.scal[S_MINRATIO];
+  int buf = 0;
+  cplx* Gb[2] = {c.mat[M_G], c.mat[M_GALT]};
+
+  for (int i = 0; i < N; ++i) {
+    if (tid == 0) {
+      sh.accepted = 0;
+      sh.stop = 0;
+      if (cur + 4 > c.n_uniforms) {
+        sh.stop = 1;
+      } else {
+        const cplx* G = Gb[buf];
+        const double* u = c.uniforms + cur;
+        double old[3], nw[3];
+        for (int a = 0; a < 3; ++a) {
+          old[a] = phl[3 * i + a];
+          nw[a] = old[a] + (-box + 2.0 * box * u[a]);
+        }
+        // calc_boson_action_diff (action.jl:57-101)
+        const double old_sq = old[0] * old[0] + old[1] * old[1] + old[2] * old[2];
+        const double new_sq = nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2];
+        const double sq_diff = new_sq - old_sq;
+        const double pow4_diff = new_sq * new_sq - old_sq * old_sq;
+        double dt_dot = 0.0, ds_dot = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          const double diff = nw[a] - old[a];
+          const double tnb = pla[3 * i + a] + phe[3 * i + a];
+          double snb = 0.0;
+          for (int k = 0; k < 4; ++k) snb += phl[3 * nbl[k * N + i] + a];
+          dt_dot += tnb * diff;
+          ds_dot += snb * diff;
+        }
+        double dS;
+        if (!c.edrun) {
+          dS = 1.0 / (c.dtau * c.c * c.c) * (sq_diff - dt_dot);
+          dS += 0.5 * c.dtau * (4 * sq_diff - 2.0 * ds_dot);
+          dS += c.dtau * (0.5 * c.r * sq_diff + 0.25 * c.u * pow4_diff);
+        } else {
+          dS = c.dtau * (0.5 * c.r * sq_diff);
+        }
+        const double exp_dS = exp(-dS);
+        // calc_detratio (local_updates.jl:42-59)
+        cplx E1[16], E2[16], Dm[16], Mm[16];
+        ev_op(lam_dtau, old[0], old[1], old[2], -1.0, E1);
+        ev_op(lam_dtau, nw[0], nw[1], nw[2], 1.0, E2);
+        mul4(E1, E2, Dm);
+        for (int a = 0; a < 4; ++a) Dm[a * 4 + a].re -= 1.0;
+        cplx Gm[16];
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) {
+            const cplx gv = G[(size_t)(i + b * N) * n + (i + a * N)];
+            Gm[a * 4 + b] = mk((a == b ? 1.0 : 0.0) - gv.re, -gv.im);  // I - G_ii
+          }
+        mul4(Dm, Gm, Mm);
+        for (int a = 0; a < 4; ++a) Mm[a * 4 + a].re += 1.0;
+        const cplx det = det4(Mm);
+        if (fabs(det.im / det.re) > 1e-4) ++nonreal;
+        const double p_acc = exp_dS * det.re;
+        int acc;
+        double uval = -1.0;
+        if (p_acc > 1.0) {
+          acc = 1;
+          cur += 3;
+        } else {
+          uval = u[3];
+          acc = uval < p_acc;
+          cur += 4;
+          minratio = fmin(minratio, fabs(p_acc - uval));
+        }
+        if (dolog && w == 0 && logi < c.log_cap) {
+          double* lg = c.prop_log + 4 * logi;
+          lg[0] = p_acc;
+          lg[1] = uval;
+          lg[2] = acc;
+          lg[3] = det.im / det.re;
+        }
+        ++logi;
+        if (acc) {
+          ++nacc;
+          dS_acc += -log(exp_dS);
+          for (int a = 0; a < 3; ++a) phl[3 * i + a] = nw[a];
+          for (int e = 0; e < 16; ++e) sh.delta[e] = Dm[e];
+          inv4(Mm, sh.minv);
+          sh.accepted = 1;
+        }
+      }
+    }
+    __syncthreads();
+    if (sh.stop) break;
+    if (!sh.accepted) continue;
+    const cplx* Gc = Gb[buf];
+    cplx* Gn = Gb[buf ^ 1];
+    // A = (G[:, c_i] - E) M^-1   (n x 4)
+    for (int r = tid; r < n; r += blockDim.x) {
+      cplx g4[4];
+      for (int k = 0; k < 4; ++k) {
+        g4[k] = Gc[(size_t)(i + k * N) * n + r];
+        if (r == i + k * N) g4[k].re -= 1.0;
+      }
+      for (int k = 0; k < 4; ++k) {
+        cplx acc = mk(0, 0);
+        for (int m = 0; m < 4; ++m) cfma(acc, g4[m], sh.minv[m * 4 + k]);
+        Acol[4 * (size_t)r + k] = acc;
+      }
+    }
+    // B = delta G[r_i, own columns]   (4 x ncol)
+    for (int jl = tid; jl < ncol; jl += blockDim.x) {
+      const int j = cb0 + jl;
+      cplx g4[4];
+      for (int k = 0; k < 4; ++k) g4[k] = Gc[(size_t)j * n + (i + k * N)];
+      for (int k = 0; k < 4; ++k) {
+        cplx acc = mk(0, 0);
+        for (int m = 0; m < 4; ++m) cfma(acc, sh.delta[k * 4 + m], g4[m]);
+        Brow[(size_t)k * ncol + jl] = acc;
+      }
+    }
+    __syncthreads();
+    for (size_t e = tid; e < (size_t)ncol * n; e += blockDim.x) {
+      const int jl = (int)(e / n), r = (int)(e - (size_t)jl * n);
+      const size_t o = (size_t)(cb0 + jl) * n + r;
+      cplx acc = Gc[o];
+      const cplx* a = Acol + 4 * (size_t)r;
+      for (int k = 0; k < 4; ++k) cfma(acc, a[k], Brow[(size_t)k * ncol + jl]);
+      Gn[o] = acc;
+    }
+    buf ^= 1;
+    group_sync(g);
+    if (group_failed(g)) return;
+  }
+  if (buf == 1) {
+    for (size_t e = tid; e < (size_t)ncol * n; e += blockDim.x) {
+      const size_t o = (size_t)cb0 * n + e;
+      Gb[0][o] = Gb[1][o];
+    }
+  }
+  if (w == 0) {
+    for (int e = tid; e < 3 * N; e += blockDim.x) c.phi[(size_t)3 * N * l + e] = phl[e];
+    if (tid == 0) {
+      *c.ucursor = cur;
+      if (dolog) *c.log_count = logi;
+      c.scal[S_BOSON_ACTION] += dS_acc;
+      c.scal[S_ACC_SLICE] = nacc;
+      c.scal[S_ACC_TOTAL] += nacc;
+      c.scal[S_PROP_TOTAL] += N;
+      c.scal[S_NONREAL] += nonreal;
+      c.scal[S_MINRATIO] = minratio;
+      if (sh.stop) *c.error = 2;
+    }
+  }
+}
+
+hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
+                                const int* slices, double box, int wg, int log) {
+  const int Nmax = nmax / 4;
+  const size_t lds = (size_t)(9 * Nmax + 2 * Nmax + 1) * 8 + (size_t)4 * nmax * 16 +
+                     (size_t)4 * ((nmax + wg - 1) / wg + 1) * 16;
+  if (lds > 150 * 1024) return hipErrorInvalidValue;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e =
+        hipFuncSetAttribute((const void*)k_local_updates, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log};
+  return hipLaunchCooperativeKernel((const void*)k_local_updates, dim3(wg, nchains), dim3(256), args, lds, s);
+}
+
+// ---------------------------------------------------------------------------
+// Single-proposal surfaces: calc_detratio (local_updates.jl:42-59) and a forced
+// update_greens! (local_updates.jl:61-95) with the stored delta and M^-1.
+// out[0..1] = det, out[2..33] = delta (column-major 4x4, interleaved),
+// out[34..65] = M (column-major 4x4, interleaved).
+// ---------------------------------------------------------------------------
+__global__ void k_detratio(const ChainDesc* chains, int slice, int site, const double* new_op, double* out) {
+  const ChainDesc& c = chains[0];
+  if (threadIdx.x != 0) return;
+  const int N = c.N, n = c.n, i = site;
+  const double* f = c.phi + 3 * ((size_t)i + (size_t)N * slice);
+  cplx E1[16], E2[16], Dm[16], Mm[16], Gm[16];
+  ev_op(c.lam * c.dtau, f[0], f[1], f[2], -1.0, E1);
+  ev_op(c.lam * c.dtau, new_op[0], new_op[1], new_op[2], 1.0, E2);
+  mul4(E1, E2, Dm);
+  for (int a = 0; a < 4; ++a) Dm[a * 4 + a].re -= 1.0;
+  const cplx* G = c.mat[M_G];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      const cplx gv = G[(size_t)(i + b * N) * n + (i + a * N)];
+      Gm[a * 4 + b] = mk((a == b ? 1.0 : 0.0) - gv.re, -gv.im);
+    }
+  mul4(Dm, Gm, Mm);
+  for (int a = 0; a < 4; ++a) Mm[a * 4 + a].re += 1.0;
+  const cplx det = det4(Mm);
+  out[0] = det.re;
+  out[1] = det.im;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      out[2 + 2 * (b * 4 + a)] = Dm[a * 4 + b].re;
+      out[3 + 2 * (b * 4 + a)] = Dm[a * 4 + b].im;
+      out[34 + 2 * (b * 4 + a)] = Mm[a * 4 + b].re;
+      out[35 + 2 * (b * 4 + a)] = Mm[a * 4 + b].im;
+    }
+  cplx Mi[16];
+  inv4(Mm, Mi);
+  for (int e = 0; e < 16; ++e) {
+    c.lu_small[e] = Dm[e];
+    c.lu_small[16 + e] = Mi[e];
+  }
+}
+
+hipError_t launch_detratio(hipStream_t s, const ChainDesc* chains, int slice, int site, const double* new_op,
+                           double* out) {
+  hipLaunchKernelGGL(k_detratio, dim3(1), dim3(64), 0, s, chains, slice, site, new_op, out);
+  return hipGetLastError();
+}
+
+// G_alt = G + (G[:, c_i] - E) M^-1 delta G[r_i, :], then copied back (one workgroup per column tile)
+__global__ __launch_bounds__(256) void k_force_update(const ChainDesc* chains, int site, int phase) {
+  const ChainDesc& c = chains[0];
+  const int n = c.n, N = c.N, i = site;
+  const cplx* G = c.mat[M_G];
+  cplx* Gn = c.mat[M_GALT];
+  const cplx* Dm = c.lu_small;
+  const cplx* Mi = c.lu_small + 16;
+  const size_t nn = (size_t)n * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+    if (phase == 1) {
+      c.mat[M_G][e] = Gn[e];
+      continue;
+    }
+    const int j = (int)(e / n), r = (int)(e - (size_t)j * n);
+    cplx g4[4], a[4], b[4], gr[4];
+    for (int k = 0; k < 4; ++k) {
+      g4[k] = G[(size_t)(i + k * N) * n + r];
+      if (r == i + k * N) g4[k].re -= 1.0;
+      gr[k] = G[(size_t)j * n + (i + k * N)];
+    }
+    for (int k = 0; k < 4; ++k) {
+      a[k] = mk(0, 0);
+      b[k] = mk(0, 0);
+      for (int m = 0; m < 4; ++m) {
+        cfma(a[k], g4[m], Mi[m * 4 + k]);
+        cfma(b[k], Dm[k * 4 + m], gr[m]);
+      }
+    }
+    cplx acc = G[e];
+    for (int k = 0; k < 4; ++k) cfma(acc, a[k], b[k]);
+    Gn[e] = acc;
+  }
+}
+
+hipError_t launch_force_update(hipStream_t s, const ChainDesc* chains, int nmax, int site) {
+  int blocks = (nmax * nmax + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_force_update, dim3(blocks), dim3(256), 0, s, chains, site, 0);
+  hipLaunchKernelGGL(k_force_update, dim3(blocks), dim3(256), 0, s, chains, site, 1);
+  return hipGetLastError();
+}
+
+__global__ void k_reset_stats(const ChainDesc* chains) {
+  const ChainDesc& c = chains[blockIdx.x];
+  if (threadIdx.x == 0) {
+    c.scal[S_ACC_SLICE] = 0;
+    c.scal[S_ACC_TOTAL] = 0;
+    c.scal[S_PROP_TOTAL] = 0;
+    c.scal[S_DRIFT_LAST] = 0;
+    c.scal[S_DRIFT_MAX] = 0;
+    c.scal[S_NONREAL] = 0;
+    c.scal[S_MINRATIO] = 1e300;
+    *c.log_count = 0;
+  }
+}
+hipError_t launch_reset_stats(hipStream_t s, const ChainDesc* chains, int nchains) {
+  hipLaunchKernelGGL(k_reset_stats, dim3(nchains), dim3(64), 0, s, chains);
+  return hipGetLastError();
+}
+
+}  // namespace dqmc

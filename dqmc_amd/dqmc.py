@@ -1,0 +1,409 @@
+"""Host-side mirror of the reference's DQMC interface over the HIP library.
+
+Names follow the reference (dqmc_framework.jl, stack.jl, local_updates.jl,
+slice_matrices.jl): `DQMC(p)`, `init(conf)` (= init!), `build_stack`,
+`propagate`, `local_updates`, `wrap_greens`, `calculate_greens`,
+`calc_greens_and_logdet`, `calc_detratio`, `update_greens`. Every call goes
+through the C ABI of libdqmc_hip.so; nothing here computes on the host beyond
+setup-time operator construction and the host uniform stream.
+
+Slices are 1-based wherever the reference uses `current_slice` (0..M+1);
+sites are 0-based.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, dptr, iptr
+from .model import Params, build_operators
+
+
+class UniformStream:
+    """The host uniform stream the reference draws with rand() (local_updates.jl:9,31).
+
+    numpy MT19937 doubles in [0, 1); `take(count)` returns the next `count` values
+    without consuming them, `advance(k)` consumes k. Any stream consumed in the same
+    order reproduces the same accept/reject sequence.
+    """
+
+    def __init__(self, seed=None, values=None):
+        self._gen = None if values is not None else np.random.Generator(np.random.MT19937(seed))
+        self._buf = np.zeros(0) if values is None else np.asarray(values, dtype=np.float64)
+        self.pos = 0  # absolute position of _buf[0]
+        self.consumed = 0
+
+    def take(self, count):
+        off = self.consumed - self.pos
+        need = off + count - len(self._buf)
+        if need > 0:
+            if self._gen is None:
+                raise ValueError("uniform stream exhausted")
+            self._buf = np.concatenate([self._buf, self._gen.random(max(need, 1 << 16))])
+        return self._buf[off:off + count]
+
+    def advance(self, k):
+        self.consumed += k
+        drop = self.consumed - self.pos
+        if drop > (1 << 20):
+            self._buf = self._buf[drop:]
+            self.pos += drop
+
+
+def _operators_struct(ops, keep):
+    s = _capi.Operators()
+    s.neighbors = iptr(ops.neighbors)
+    keep.append(ops.neighbors)
+    if ops.plaq is not None:
+        ps = np.ascontiguousarray(ops.plaq_sites, dtype=np.int32)
+        pl = np.ascontiguousarray(ops.plaq, dtype=np.complex128)
+        em = np.ascontiguousarray(ops.expmu, dtype=np.float64)
+        keep += [ps, pl, em]
+        s.plaq_sites = iptr(ps)
+        s.plaq = pl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        s.expmu = dptr(em)
+    if ops.hop_exp is not None:
+        he = np.asfortranarray(ops.hop_exp, dtype=np.complex128)
+        hi = np.asfortranarray(ops.hop_exp_inv, dtype=np.complex128)
+        keep += [he, hi]
+        s.hop_exp = he.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        s.hop_exp_inv = hi.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    return s
+
+
+def _params_struct(p):
+    return _capi.Params(L=p.L, slices=p.slices, safe_mult=p.safe_mult, checkerboard=int(p.checkerboard),
+                        edrun=int(p.edrun), all_checks=int(p.all_checks), delta_tau=p.delta_tau, lam=p.lam,
+                        r=p.r, c=p.c, u=p.u)
+
+
+def _cmat_ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _as_fcomplex(a, n):
+    a = np.asfortranarray(a, dtype=np.complex128)
+    if a.shape != (n, n):
+        raise ValueError("expected a %dx%d matrix" % (n, n))
+    return a
+
+
+class DQMC:
+    """One Markov chain on the device (DQMC(p) + initialize_stack, dqmc_framework.jl:105-135)."""
+
+    def __init__(self, p: Params, device=0, uniforms=None, ops=None):
+        self.p = p
+        self.lib = _capi.load()
+        self._keep = []
+        ops = ops if ops is not None else build_operators(p)
+        self._ops_struct = _operators_struct(ops, self._keep)
+        self._params_struct = _params_struct(p)
+        h = ctypes.c_void_p()
+        check(self.lib.dqmc_ctx_create(ctypes.byref(self._params_struct), ctypes.byref(self._ops_struct),
+                                       device, ctypes.byref(h)))
+        self.h = h
+        self.stream = uniforms if uniforms is not None else UniformStream(p.seed)
+        self._log_cap = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dqmc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- field ----------------------------------------------------------------
+    @property
+    def hsfield(self):
+        phi = np.zeros((3, self.p.N, self.p.slices), order="F")
+        check(self.lib.dqmc_get_field(self.h, dptr(phi)))
+        return phi
+
+    @hsfield.setter
+    def hsfield(self, phi):
+        phi = np.asfortranarray(phi, dtype=np.float64)
+        if phi.shape != (3, self.p.N, self.p.slices):
+            raise ValueError("hsfield must have shape (3, N, M)")
+        check(self.lib.dqmc_set_field(self.h, dptr(phi)))
+
+    # -- init!, stack, propagation --------------------------------------------
+    def init(self, conf):
+        """init!(mc, conf) (dqmc_framework.jl:156-177)."""
+        self.hsfield = conf
+        check(self.lib.dqmc_init(self.h))
+
+    def build_stack(self):
+        check(self.lib.dqmc_build_stack(self.h))
+
+    def propagate(self, want_drift=False):
+        """propagate (stack.jl:391-499). Returns (current_slice, direction, drift or None)."""
+        cs, d = ctypes.c_int(), ctypes.c_int()
+        dr = ctypes.c_double(-1.0)
+        check(self.lib.dqmc_propagate(self.h, ctypes.byref(cs), ctypes.byref(d),
+                                      ctypes.byref(dr) if want_drift else None))
+        drift = dr.value if (want_drift and dr.value >= 0) else None
+        return cs.value, d.value, drift
+
+    def _upload_uniforms(self, count):
+        vals = np.ascontiguousarray(self.stream.take(count))
+        check(self.lib.dqmc_set_uniforms(self.h, dptr(vals), len(vals)))
+
+    def _consume_uniforms(self):
+        st = self.stats()
+        self.stream.advance(st.uniforms_used)
+
+    def local_updates(self, box=None):
+        """local_updates (local_updates.jl:1-39); returns the acceptance rate."""
+        box = self.p.box if box is None else box
+        self._upload_uniforms(4 * self.p.N)
+        acc = ctypes.c_double()
+        check(self.lib.dqmc_local_updates(self.h, box, ctypes.byref(acc)))
+        self._consume_uniforms()
+        return acc.value
+
+    def update(self, box=None):
+        """update (dqmc_framework.jl:500-517) without global updates."""
+        self.propagate()
+        return self.local_updates(box)
+
+    def sweep(self, nsweeps=1, box=None, sync=True):
+        """nsweeps x M update steps without host round trips between slices."""
+        box = self.p.box if box is None else box
+        self._upload_uniforms(4 * self.p.N * self.p.slices * nsweeps)
+        check(self.lib.dqmc_sweep(self.h, nsweeps, box))
+        if sync:
+            self.synchronize()
+            self._consume_uniforms()
+
+    def synchronize(self):
+        check(self.lib.dqmc_synchronize(self.h))
+
+    def set_groups(self, wg_qr=0, wg_inv=0, wg_local=0):
+        check(self.lib.dqmc_ctx_set_groups(self.h, wg_qr, wg_inv, wg_local))
+
+    # -- state ---------------------------------------------------------------
+    @property
+    def greens(self):
+        G = np.zeros((self.p.n, self.p.n), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_get_greens(self.h, _cmat_ptr(G)))
+        return G
+
+    @greens.setter
+    def greens(self, G):
+        G = _as_fcomplex(G, self.p.n)
+        check(self.lib.dqmc_set_greens(self.h, _cmat_ptr(G)))
+
+    def stack_element(self, k):
+        n = self.p.n
+        U = np.zeros((n, n), dtype=np.complex128, order="F")
+        T = np.zeros((n, n), dtype=np.complex128, order="F")
+        D = np.zeros(n)
+        check(self.lib.dqmc_get_stack(self.h, k, _cmat_ptr(U), dptr(D), _cmat_ptr(T)))
+        return U, D, T
+
+    def stats(self):
+        st = _capi.Stats()
+        check(self.lib.dqmc_get_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def reset_stats(self):
+        check(self.lib.dqmc_reset_stats(self.h))
+
+    @property
+    def current_slice(self):
+        return self.stats().current_slice
+
+    @property
+    def direction(self):
+        return self.stats().direction
+
+    @property
+    def boson_action(self):
+        return self.stats().boson_action
+
+    @property
+    def log_det(self):
+        return self.stats().log_det
+
+    def set_proposal_log(self, capacity):
+        self._log_cap = capacity
+        check(self.lib.dqmc_set_proposal_log(self.h, capacity))
+
+    def proposal_log(self):
+        cnt = ctypes.c_longlong()
+        out = np.zeros((max(self._log_cap, 1), 4))
+        check(self.lib.dqmc_get_proposal_log(self.h, dptr(out), ctypes.byref(cnt)))
+        return out[:min(cnt.value, self._log_cap)]
+
+    # -- unit surfaces --------------------------------------------------------
+    def _apply(self, slice_, which, M):
+        X = _as_fcomplex(M, self.p.n).copy(order="F")
+        check(self.lib.dqmc_apply_B(self.h, slice_ - 1, which, _cmat_ptr(X)))
+        return X
+
+    def multiply_B_left(self, slice_, M):
+        return self._apply(slice_, 0, M)
+
+    def multiply_B_right(self, slice_, M):
+        return self._apply(slice_, 1, M)
+
+    def multiply_B_inv_left(self, slice_, M):
+        return self._apply(slice_, 2, M)
+
+    def multiply_B_inv_right(self, slice_, M):
+        return self._apply(slice_, 3, M)
+
+    def multiply_daggered_B_left(self, slice_, M):
+        return self._apply(slice_, 4, M)
+
+    def slice_matrix(self, slice_, power=1.0):
+        eye = np.eye(self.p.n, dtype=np.complex128)
+        return self.multiply_B_left(slice_, eye) if power > 0 else self.multiply_B_inv_left(slice_, eye)
+
+    def wrap_greens(self, G, slice_, direction):
+        X = _as_fcomplex(G, self.p.n).copy(order="F")
+        check(self.lib.dqmc_wrap_greens(self.h, _cmat_ptr(X), slice_, direction))
+        return X
+
+    def decompose_udt(self, A):
+        n = self.p.n
+        A = _as_fcomplex(A, n)
+        U = np.zeros((n, n), dtype=np.complex128, order="F")
+        T = np.zeros((n, n), dtype=np.complex128, order="F")
+        D = np.zeros(n)
+        check(self.lib.dqmc_udt(self.h, _cmat_ptr(A), _cmat_ptr(U), dptr(D), _cmat_ptr(T)))
+        return U, D, T
+
+    def calculate_greens(self, Ul, Dl, Tl, Ur, Dr, Tr):
+        """calculate_greens + calculate_logdet (stack.jl:338-385) on given factors."""
+        n = self.p.n
+        args = [_as_fcomplex(Ul, n), np.ascontiguousarray(Dl, dtype=np.float64), _as_fcomplex(Tl, n),
+                _as_fcomplex(Ur, n), np.ascontiguousarray(Dr, dtype=np.float64), _as_fcomplex(Tr, n)]
+        G = np.zeros((n, n), dtype=np.complex128, order="F")
+        ld = ctypes.c_double()
+        check(self.lib.dqmc_calculate_greens(self.h, _cmat_ptr(args[0]), dptr(args[1]), _cmat_ptr(args[2]),
+                                             _cmat_ptr(args[3]), dptr(args[4]), _cmat_ptr(args[5]),
+                                             _cmat_ptr(G), ctypes.byref(ld)))
+        return G, ld.value
+
+    def calc_greens_and_logdet(self, slice_):
+        """fermion_measurements.jl:1068-1101 (G(slice) from scratch)."""
+        n = self.p.n
+        G = np.zeros((n, n), dtype=np.complex128, order="F")
+        ld = ctypes.c_double()
+        check(self.lib.dqmc_greens_at(self.h, slice_, _cmat_ptr(G), ctypes.byref(ld)))
+        return G, ld.value
+
+    def calc_detratio(self, site, new_op):
+        """calc_detratio (local_updates.jl:42-59): (det, delta, M)."""
+        op = np.ascontiguousarray(new_op, dtype=np.float64)
+        det = np.zeros(2)
+        delta = np.zeros((4, 4), dtype=np.complex128, order="F")
+        M = np.zeros((4, 4), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_detratio(self.h, site, dptr(op), dptr(det), _cmat_ptr(delta), _cmat_ptr(M)))
+        return complex(det[0], det[1]), delta, M
+
+    def update_greens(self, site):
+        """update_greens! (local_updates.jl:61-95) with the last calc_detratio's delta and M."""
+        check(self.lib.dqmc_update_greens(self.h, site))
+
+    def gemm(self, A, B, opA=0, opB=0):
+        """mul!(C, op(A), op(B)) on the device; op 1 = adjoint."""
+        n = self.p.n
+        A = _as_fcomplex(A, n)
+        B = _as_fcomplex(B, n)
+        C = np.zeros((n, n), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_gemm(self.h, opA, opB, _cmat_ptr(A), _cmat_ptr(B), _cmat_ptr(C)))
+        return C
+
+    def inv(self, A):
+        """(inv(A), log|det A|) on the device."""
+        n = self.p.n
+        A = _as_fcomplex(A, n)
+        X = np.zeros((n, n), dtype=np.complex128, order="F")
+        ld = ctypes.c_double()
+        check(self.lib.dqmc_inverse(self.h, _cmat_ptr(A), _cmat_ptr(X), ctypes.byref(ld)))
+        return X, ld.value
+
+    def calc_boson_action(self):
+        s = ctypes.c_double()
+        check(self.lib.dqmc_boson_action(self.h, ctypes.byref(s)))
+        return s.value
+
+    def grad(self):
+        """dS/dphi (3, N, M) and S = S_B + log|det G| of the current field."""
+        g = np.zeros((3, self.p.N, self.p.slices), order="F")
+        s = ctypes.c_double()
+        check(self.lib.dqmc_grad(self.h, dptr(g), ctypes.byref(s)))
+        return g, s.value
+
+
+class Batch:
+    """Many independent chains (possibly of different L, M) advanced together."""
+
+    def __init__(self, params_list, device=0, streams=None):
+        self.lib = _capi.load()
+        self.params = list(params_list)
+        self._keep = []
+        nc = len(self.params)
+        P = (_capi.Params * nc)(*[_params_struct(p) for p in self.params])
+        O = (_capi.Operators * nc)(*[_operators_struct(build_operators(p), self._keep) for p in self.params])
+        h = ctypes.c_void_p()
+        check(self.lib.dqmc_batch_create(P, O, nc, device, ctypes.byref(h)))
+        self.h = h
+        self.streams = streams if streams is not None else [UniformStream(p.seed) for p in self.params]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dqmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_field(self, chain, phi):
+        phi = np.asfortranarray(phi, dtype=np.float64)
+        check(self.lib.dqmc_batch_set_field(self.h, chain, dptr(phi)))
+
+    def get_field(self, chain):
+        p = self.params[chain]
+        phi = np.zeros((3, p.N, p.slices), order="F")
+        check(self.lib.dqmc_batch_get_field(self.h, chain, dptr(phi)))
+        return phi
+
+    def init(self, confs):
+        for c, conf in enumerate(confs):
+            self.set_field(c, conf)
+        check(self.lib.dqmc_batch_init(self.h))
+
+    def sweep(self, nsweeps=1, sync=True):
+        for c, p in enumerate(self.params):
+            vals = np.ascontiguousarray(self.streams[c].take(4 * p.N * p.slices * nsweeps))
+            check(self.lib.dqmc_batch_set_uniforms(self.h, c, dptr(vals), len(vals)))
+        box = np.array([p.box for p in self.params])
+        check(self.lib.dqmc_batch_sweep(self.h, nsweeps, dptr(box)))
+        if sync:
+            self.synchronize()
+            for c in range(len(self.params)):
+                self.streams[c].advance(self.stats(c).uniforms_used)
+
+    def synchronize(self):
+        check(self.lib.dqmc_batch_synchronize(self.h))
+
+    def greens(self, chain):
+        n = self.params[chain].n
+        G = np.zeros((n, n), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_batch_get_greens(self.h, chain, _cmat_ptr(G)))
+        return G
+
+    def stats(self, chain):
+        st = _capi.Stats()
+        check(self.lib.dqmc_batch_get_stats(self.h, chain, ctypes.byref(st)))
+        return st

@@ -1,0 +1,189 @@
+/*
+ * dqmc_hip.h -- C ABI of the MI355X DQMC sweep library (libdqmc_hip.so).
+ *
+ * Drop-in boundary for the DQMC sweep hot path of carstenbauer/dqmc (O(3)
+ * spin-fermion model). Every entry point replaces one reference function; the
+ * replaced function is cited as file:line of the reference sources. Matrices
+ * cross the boundary as column-major complex128 arrays with interleaved
+ * (re, im) doubles, i.e. Julia's Matrix{ComplexF64} memory layout; the field
+ * phi is the reference's (3, N, M) Float64 array. All indices are 0-based.
+ *
+ * Conventions:
+ *   - every function returns 0 on success and a negative DQMC_E* code on error;
+ *     dqmc_last_error() then describes the failure (thread-local string);
+ *   - host buffers are owned by the caller and only accessed during the call;
+ *   - a context (one chain) or batch (many chains) must be used from one host
+ *     thread at a time; different contexts are independent;
+ *   - calls are asynchronous on the context's HIP stream unless they return
+ *     data to the host (getters, optional out-parameters), which synchronise.
+ */
+#ifndef DQMC_HIP_H
+#define DQMC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DQMC_OK 0
+#define DQMC_E_ARG (-1)     /* invalid argument or size */
+#define DQMC_E_HIP (-2)     /* HIP runtime error */
+#define DQMC_E_STATE (-3)   /* call not valid in the current state */
+#define DQMC_E_DEVICE (-4)  /* device-side failure (barrier timeout, uniform stream exhausted) */
+
+typedef struct dqmc_ctx dqmc_ctx;
+typedef struct dqmc_batch dqmc_batch;
+
+/* Model parameters (reference Params, parameters.jl:4-80). The lattice is the
+ * L x L periodic square lattice, N = L*L, n = 4N (flv = 4, opdim = 3). */
+typedef struct {
+  int L;
+  int slices;       /* M = beta / delta_tau */
+  int safe_mult;    /* stabilisation interval; slices % safe_mult == 0 */
+  int checkerboard; /* 1: Assaad 4-site checkerboard (CBAssaad), 0: dense (CBFalse) */
+  int edrun;        /* 1: bosonic action has only the mass term (parameters.jl:50) */
+  int all_checks;   /* 1: record max|G_wrapped - G_fresh| at each recompute */
+  double delta_tau, lambda, r, c, u;
+} dqmc_params;
+
+/* Host-computed hopping operators (the reference's setup code produces them:
+ * init_checkerboard_matrices(_Bfield), hoppings_checkerboard.jl:65-136/199-270,
+ * and init_hopping_matrix_exp(_Bfield), hoppings.jl:16-95/158-254).
+ *
+ * Checkerboard: chkr_hop_half[g], chkr_hop_half_inv[g], chkr_hop[g],
+ * chkr_hop_inv[g] (g = A, B) are block-diagonal over the 4 flavour blocks and,
+ * within a block, over disjoint 4-site plaquettes; they are passed as the list
+ * of 4x4 plaquette blocks:
+ *   plaq_sites[g][p][k]                      site of corner k of plaquette p of group g
+ *   plaq[v][g][b][p][r][c] (complex, interleaved)  entry (r, c) of the block of
+ *       variant v (0 half, 1 half_inv, 2 full, 3 full_inv), group g, flavour block b
+ * expmu[4N] is the diagonal of chkr_mu = exp(+dtau mu).
+ * Dense (checkerboard = 0): hop_exp / hop_exp_inv are hopping_matrix_exp(_inv)
+ * as n x n complex column-major. neighbors[k][i] (k = up, right, down, left)
+ * is the reference's neighbor table (lattice.jl:105-117) transposed, 0-based. */
+typedef struct {
+  const int* plaq_sites;
+  const double* plaq;
+  const double* expmu;
+  const double* hop_exp;
+  const double* hop_exp_inv;
+  const int* neighbors;
+} dqmc_operators;
+
+/* Counters and diagnostics of a chain. */
+typedef struct {
+  int current_slice;         /* 1-based like mc.s.current_slice, 0..M+1 */
+  int direction;             /* +1 / -1 like mc.s.direction */
+  double boson_action;       /* p.boson_action */
+  double log_det;            /* mc.s.log_det (after calculate_logdet) */
+  double drift_last;         /* max|G_wrapped - G_fresh| at the last recompute (stack.jl:424-429, 475-480) */
+  double drift_max;          /* running maximum since the last stats reset */
+  double acc_last;           /* acceptance rate of the last local_updates call */
+  double accepted;           /* accepted proposals since reset */
+  double proposed;           /* proposals since reset */
+  double nonreal;            /* proposals with |Im r/Re r| > 1e-4 (local_updates.jl:19-20) */
+  double min_ratio_gap;      /* min |p_acc - u| over proposals that drew u */
+  long long uniforms_used;   /* position of the uniform stream cursor */
+  int device_error;          /* nonzero after a device-side failure */
+} dqmc_stats;
+
+const char* dqmc_last_error(void);
+
+/* ---- single chain ------------------------------------------------------- */
+
+/* DQMC(p) + initialize_stack (dqmc_framework.jl:105-135, stack.jl:224-242). */
+int dqmc_ctx_create(const dqmc_params* p, const dqmc_operators* ops, int device, dqmc_ctx** out);
+int dqmc_ctx_destroy(dqmc_ctx* ctx);
+/* workgroups per chain for the cooperative kernels (QR, inverse, local updates); 0 = default */
+int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local);
+
+/* p.hsfield (parameters.jl:19): phi[a + 3*(i + N*l)] */
+int dqmc_set_field(dqmc_ctx* ctx, const double* phi);
+int dqmc_get_field(dqmc_ctx* ctx, double* phi);
+
+/* The uniform stream the reference draws with rand() (local_updates.jl:9,31):
+ * the device consumes it in order, 3 draws per proposal plus 1 when p_acc <= 1.
+ * Uploading resets the cursor to 0. */
+int dqmc_set_uniforms(dqmc_ctx* ctx, const double* u, long long count);
+
+/* init!(mc, conf) after dqmc_set_field (dqmc_framework.jl:156-177):
+ * boson action, build_stack, propagate. */
+int dqmc_init(dqmc_ctx* ctx);
+/* build_stack (stack.jl:251-272) */
+int dqmc_build_stack(dqmc_ctx* ctx);
+/* propagate (stack.jl:391-499); out-params optional (NULL), drift = -1 when no recompute */
+int dqmc_propagate(dqmc_ctx* ctx, int* current_slice, int* direction, double* drift);
+/* local_updates (local_updates.jl:1-39) at the current slice with box half-width `box`;
+ * acc_rate optional (synchronises). */
+int dqmc_local_updates(dqmc_ctx* ctx, double box, double* acc_rate);
+/* `nsweeps` x M update steps (propagate + local_updates, dqmc_framework.jl:258-261 / 500-517
+ * without global updates); one sweep = M steps = the reference's printed "sweep duration". */
+int dqmc_sweep(dqmc_ctx* ctx, int nsweeps, double box);
+int dqmc_synchronize(dqmc_ctx* ctx);
+
+/* mc.s.greens */
+int dqmc_get_greens(dqmc_ctx* ctx, double* G);
+int dqmc_set_greens(dqmc_ctx* ctx, const double* G);
+/* u_stack/d_stack/t_stack element k (stack.jl:87-89); any pointer may be NULL */
+int dqmc_get_stack(dqmc_ctx* ctx, int k, double* U, double* D, double* T);
+int dqmc_get_stats(dqmc_ctx* ctx, dqmc_stats* st);
+int dqmc_reset_stats(dqmc_ctx* ctx);
+/* per-proposal log (p_acc, u or -1, accepted, Im r/Re r) of the next local updates;
+ * capacity in proposals, 0 disables */
+int dqmc_set_proposal_log(dqmc_ctx* ctx, long long capacity);
+int dqmc_get_proposal_log(dqmc_ctx* ctx, double* out, long long* count);
+
+/* Unit surfaces (reference functions usable on host data) --------------- */
+/* multiply_B_left!/right!/inv_left!/inv_right!, multiply_daggered_B_left!
+ * (slice_matrices.jl:101-226 / 25-94); which = 0 left, 1 right, 2 inv_left,
+ * 3 inv_right, 4 daggered_left; X is n x n, updated in place. */
+int dqmc_apply_B(dqmc_ctx* ctx, int slice, int which, double* X);
+/* wrap_greens (stack.jl:316-332): G <- wrapped G for (slice 1-based, direction) */
+int dqmc_wrap_greens(dqmc_ctx* ctx, double* G, int slice, int direction);
+/* decompose_udt (linalg.jl:10-39) of an n x n matrix */
+int dqmc_udt(dqmc_ctx* ctx, const double* A, double* U, double* D, double* T);
+/* calculate_greens + calculate_logdet (stack.jl:338-385) from given UDT factors */
+int dqmc_calculate_greens(dqmc_ctx* ctx, const double* Ul, const double* Dl, const double* Tl,
+                          const double* Ur, const double* Dr, const double* Tr, double* G,
+                          double* logdet);
+/* calc_greens_and_logdet (fermion_measurements.jl:1068-1101): G(slice) from scratch */
+int dqmc_greens_at(dqmc_ctx* ctx, int slice, double* G, double* logdet);
+/* calc_detratio (local_updates.jl:42-59) at the current slice; any out pointer may be NULL:
+ * det[2], delta[16 complex, column-major 4x4], M[16 complex] */
+int dqmc_detratio(dqmc_ctx* ctx, int site, const double* new_op, double* det, double* delta, double* M);
+/* update_greens! (local_updates.jl:61-95) with the delta/M of the last dqmc_detratio call */
+int dqmc_update_greens(dqmc_ctx* ctx, int site);
+/* mul!(C, op(A), op(B)) for n x n complex matrices (zgemm through LinearAlgebra.mul!,
+ * stack.jl:291,346-366); opA/opB: 0 = none, 1 = adjoint */
+int dqmc_gemm(dqmc_ctx* ctx, int opA, int opB, const double* A, const double* B, double* C);
+/* inv(A) and log|det A| (Julia inv/logdet via getrf+getri, stack.jl:359, :381-383) */
+int dqmc_inverse(dqmc_ctx* ctx, const double* A, double* Ainv, double* logabsdet);
+/* calc_boson_action (action.jl:1-52) of the current field */
+int dqmc_boson_action(dqmc_ctx* ctx, double* S);
+
+/* No reference counterpart: dS/dphi for every (component, site, slice) of
+ * S(phi) = S_B(phi) - log det(1 + B_M...B_1), computed from the Green's
+ * functions of one stabilised pass over all slices. Also returns S (optional). */
+int dqmc_grad(dqmc_ctx* ctx, double* grad, double* S);
+
+/* ---- batches of independent chains -------------------------------------- */
+/* No reference counterpart: many chains (possibly of different L, M) advanced
+ * together; chains of equal shape share every kernel launch. */
+int dqmc_batch_create(const dqmc_params* p, const dqmc_operators* ops, int nchains, int device,
+                      dqmc_batch** out);
+int dqmc_batch_destroy(dqmc_batch* b);
+int dqmc_batch_set_field(dqmc_batch* b, int chain, const double* phi);
+int dqmc_batch_get_field(dqmc_batch* b, int chain, double* phi);
+int dqmc_batch_set_uniforms(dqmc_batch* b, int chain, const double* u, long long count);
+int dqmc_batch_init(dqmc_batch* b);
+/* nsweeps sweeps of every chain; box[chain] */
+int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box);
+int dqmc_batch_synchronize(dqmc_batch* b);
+int dqmc_batch_get_greens(dqmc_batch* b, int chain, double* G);
+int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st);
+/* capture each shape group's sweep in a HIP graph and replay it (1) or launch eagerly (0) */
+int dqmc_batch_use_graph(dqmc_batch* b, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
