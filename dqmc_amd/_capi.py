@@ -87,9 +87,18 @@ SIGNATURES = {
     "dqmc_batch_get_greens": [_P, _i, _D],
     "dqmc_batch_get_stats": [_P, _i, ctypes.POINTER(Stats)],
     "dqmc_batch_use_graph": [_P, _i],
+    "dqmc_profile": [_P, ctypes.c_uint],
+    "dqmc_profile_read": [_P, _D, _LL, _i],
+    "dqmc_batch_profile": [_P, ctypes.c_uint],
+    "dqmc_batch_profile_read": [_P, _D, _LL, _i],
     "dqmc_last_error": [],
 }
 _RET = {"dqmc_last_error": ctypes.c_char_p}
+
+# kernel classes of dqmc_profile (include/dqmc_hip.h)
+K_APPLY_B, K_GEMM, K_QRP, K_QFORM, K_INVERSE, K_LOCAL, K_OTHER = range(7)
+K_NCLASS = 7
+K_NAMES = ["apply_B", "gemm", "qrp", "qform", "inverse", "local_updates", "other"]
 
 _lib = None
 

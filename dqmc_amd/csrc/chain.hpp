@@ -158,6 +158,10 @@ hipError_t launch_maxdiff(hipStream_t s, const ChainDesc* chains, int nchains, i
 // A is consumed (overwritten by the factorization). wg = workgroups per chain.
 hipError_t launch_udt(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
                       MatRef Qout, MatRef Tout, int wg);
+// the two halves of launch_udt: the pivoted QR itself, then Q and T formation
+hipError_t launch_qrp(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D, int wg);
+hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
+                          MatRef Qout, MatRef Tout);
 
 // X <- X^-1 in place (Gauss-Jordan, partial pivoting); scal[S_LOGDET_A] = log|det X|.
 hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);

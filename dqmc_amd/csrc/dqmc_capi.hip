@@ -66,6 +66,24 @@ struct Group {
   int cur = 0, dir = 1;  // reference current_slice (1-based), direction
   int wg_qr = 1, wg_inv = 1, wg_local = 1;
   bool recomputed = false;
+  // kernel-class timing (dqmc_profile)
+  unsigned prof_mask = 0;
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> prof;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
   int nch() const { return (int)chains.size(); }
 };
 
@@ -266,6 +284,26 @@ int default_groups(Group& g) {
     if (_e != hipSuccess) return fail(DQMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
+// Per-class kernel timing with HIP events on the group's stream (dqmc_profile).
+struct ProfScope {
+  Group& g;
+  int cls;
+  hipEvent_t e0 = nullptr;
+  ProfScope(Group& gr, int c) : g(gr), cls(c) {
+    if (g.prof_mask & (1u << cls)) {
+      e0 = g.take_event();
+      (void)hipEventRecord(e0, g.stream);
+    }
+  }
+  ~ProfScope() {
+    if (e0) {
+      hipEvent_t e1 = g.take_event();
+      (void)hipEventRecord(e1, g.stream);
+      g.prof.push_back({cls, e0, e1});
+    }
+  }
+};
+
 struct Ops {
   Group& g;
   explicit Ops(Group& gr) : g(gr) {}
@@ -274,28 +312,38 @@ struct Ops {
   int nc() const { return g.nch(); }
 
   int copy(MatRef dst, MatRef src) {
+    ProfScope ps(g, DQMC_K_OTHER);
     LCHK(launch_copy(s(), d(), nc(), g.n, dst, src));
     return 0;
   }
   int copyv(VecRef dst, VecRef src) {
+    ProfScope ps(g, DQMC_K_OTHER);
     LCHK(launch_copy_vec(s(), d(), nc(), g.n, dst, src));
     return 0;
   }
   int gemm(MatRef C, MatRef A, GemmOp oa, MatRef B, GemmOp ob, VecRef dl = NOVEC, VecRef dr = NOVEC) {
+    ProfScope ps(g, DQMC_K_GEMM);
     LCHK(launch_gemm(s(), d(), nc(), g.n, C, A, oa, B, ob, dl, dr, 0.0));
     return 0;
   }
   int udt(MatRef A, VecRef D, MatRef Q, MatRef T) {
-    LCHK(launch_udt(s(), d(), nc(), g.n, A, D, Q, T, g.wg_qr));
+    {
+      ProfScope ps(g, DQMC_K_QRP);
+      LCHK(launch_qrp(s(), d(), nc(), g.n, A, D, g.wg_qr));
+    }
+    ProfScope ps(g, DQMC_K_QFORM);
+    LCHK(launch_form_TQ(s(), d(), nc(), g.n, A, D, Q, T));
     return 0;
   }
   int inverse(MatRef X) {
+    ProfScope ps(g, DQMC_K_INVERSE);
     LCHK(launch_inverse(s(), d(), nc(), g.n, X, g.wg_inv));
     return 0;
   }
   // X <- op(B_slice) X / X op(B_slice); slice 0-based
   int applyB(MatRef X, BOp op, int slice) {
     if (g.chkr) {
+      ProfScope ps(g, DQMC_K_APPLY_B);
       LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, op, slice, nullptr));
       return 0;
     }
@@ -509,6 +557,7 @@ struct Ops {
 
   int local_updates(double box, int dolog) {
     if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "local_updates outside slices 1..M");
+    ProfScope ps(g, DQMC_K_LOCAL);
     LCHK(launch_local_updates(s(), d(), nc(), g.n, g.cur - 1, nullptr, box, g.wg_local, dolog));
     return 0;
   }
@@ -1106,8 +1155,10 @@ int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box) {
       const int total = nsweeps * g.M;
       if (steps[gi] >= total) continue;
       Ops o(g);
+      int dolog = 0;
+      for (int ci : g.chains) dolog |= b->chains[ci].log_cap > 0;
       RC(o.propagate());
-      RC(o.local_updates(box[g.chains[0]], 0));
+      RC(o.local_updates(box[g.chains[0]], dolog));
       steps[gi] += 1;
       more = more || steps[gi] < total;
     }
@@ -1129,6 +1180,45 @@ int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
   return get_stats(b, chain, st);
 }
+int dqmc_batch_profile(dqmc_batch* b, unsigned class_mask) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  for (auto& g : b->groups) g.prof_mask = class_mask;
+  return 0;
+}
+
+int dqmc_batch_profile_read(dqmc_batch* b, double* ms, long long* count, int nclass) {
+  if (!b || nclass < 0) return fail(DQMC_E_ARG, "bad profile args");
+  RC(batch_sync(b));
+  for (int k = 0; k < nclass; ++k) {
+    if (ms) ms[k] = 0.0;
+    if (count) count[k] = 0;
+  }
+  for (auto& g : b->groups) {
+    for (auto& r : g.prof) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+      if (r.cls < nclass) {
+        if (ms) ms[r.cls] += t;
+        if (count) count[r.cls] += 1;
+      }
+      g.event_pool.push_back(r.a);
+      g.event_pool.push_back(r.b);
+    }
+    g.prof.clear();
+  }
+  return 0;
+}
+
+int dqmc_profile(dqmc_ctx* ctx, unsigned class_mask) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_profile(ctx->b, class_mask);
+}
+
+int dqmc_profile_read(dqmc_ctx* ctx, double* ms, long long* count, int nclass) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_profile_read(ctx->b, ms, count, nclass);
+}
+
 int dqmc_batch_use_graph(dqmc_batch* b, int enable) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
   b->use_graph = enable != 0;

@@ -9,6 +9,8 @@
 // a 16x16 complex sub-tile accumulated with v_mfma_f64_16x16x4 (four real MFMAs
 // per complex k-step: re*re, -im*im, re*im, im*re). K advances in chunks of 16
 // staged through LDS.
+#include <stdlib.h>
+
 #include "chain.hpp"
 
 namespace dqmc {
@@ -17,6 +19,95 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int GT = 32;  // C tile edge
 constexpr int KT = 16;  // K chunk
+
+// Row of accumulator element v held by `lane` in v_mfma_f64_16x16x4 (column = lane % 16):
+// rows are interleaved, lane group g = lane / 16 holds rows g, g + 4, g + 8, g + 12
+// (measured on gfx950 with an in-bounds probe of the builtin).
+__device__ inline int kMfmaRow(int lane, int v) { return (lane >> 4) + 4 * v; }
+
+// Shared tile loader: As[r][kk] = op(A)[i0 + r][k0 + kk], Bs[kk][c] = op(B)[k0 + kk][j0 + c].
+template <int OPA, int OPB>
+__device__ inline void load_tiles(const cplx* A, const cplx* B, int n, int i0, int j0, int k0,
+                                  double (*As_re)[KT + 1], double (*As_im)[KT + 1], double (*Bs_re)[GT + 1],
+                                  double (*Bs_im)[GT + 1]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int e = tid + 256 * t;
+    int r, kk;
+    if (OPA == OP_N) {
+      r = e & 31;
+      kk = e >> 5;
+    } else {
+      kk = e & 15;
+      r = e >> 4;
+    }
+    const int gi = i0 + r, gk = k0 + kk;
+    cplx v = mk(0, 0);
+    if (gi < n && gk < n) v = (OPA == OP_N) ? A[(size_t)gk * n + gi] : conj(A[(size_t)gi * n + gk]);
+    As_re[r][kk] = v.re;
+    As_im[r][kk] = v.im;
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int e = tid + 256 * t;
+    int kk, cc;
+    if (OPB == OP_N) {
+      kk = e & 15;
+      cc = e >> 4;
+    } else {
+      cc = e & 31;
+      kk = e >> 5;
+    }
+    const int gk = k0 + kk, gj = j0 + cc;
+    cplx v = mk(0, 0);
+    if (gk < n && gj < n) v = (OPB == OP_N) ? B[(size_t)gj * n + gk] : conj(B[(size_t)gk * n + gj]);
+    Bs_re[kk][cc] = v.re;
+    Bs_im[kk][cc] = v.im;
+  }
+}
+
+// Vector-FMA variant (selected with DQMC_GEMM_VECTOR=1): each thread owns 4 rows of one column.
+template <int OPA, int OPB>
+__global__ __launch_bounds__(256) void k_gemm_vec(const ChainDesc* chains, MatRef Cr, MatRef Ar, MatRef Br,
+                                                  VecRef dlr, VecRef drr, double beta) {
+  const ChainDesc& c = chains[blockIdx.z];
+  const int n = c.n;
+  const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
+  if (i0 >= n || j0 >= n) return;
+  const cplx* A = matptr(c, Ar);
+  const cplx* B = matptr(c, Br);
+  cplx* C = matptr(c, Cr);
+  const double* dl = dlr.kind >= 0 ? vecptr(c, dlr) : nullptr;
+  const double* dr = drr.kind >= 0 ? vecptr(c, drr) : nullptr;
+  __shared__ double As_re[GT][KT + 1], As_im[GT][KT + 1];
+  __shared__ double Bs_re[KT][GT + 1], Bs_im[KT][GT + 1];
+  const int cc = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
+  cplx acc[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  for (int k0 = 0; k0 < n; k0 += KT) {
+    load_tiles<OPA, OPB>(A, B, n, i0, j0, k0, As_re, As_im, Bs_re, Bs_im);
+    __syncthreads();
+    for (int kk = 0; kk < KT; ++kk) {
+      const cplx b = mk(Bs_re[kk][cc], Bs_im[kk][cc]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cfma(acc[q], mk(As_re[r0 + q][kk], As_im[r0 + q][kk]), b);
+    }
+    __syncthreads();
+  }
+  const int col = j0 + cc;
+  for (int q = 0; q < 4; ++q) {
+    const int row = i0 + r0 + q;
+    if (row < n && col < n) {
+      double s = 1.0;
+      if (dl) s *= dl[row];
+      if (dr) s *= dr[col];
+      cplx val = s * acc[q];
+      const size_t o = (size_t)col * n + row;
+      if (beta != 0.0) val = val + beta * C[o];
+      C[o] = val;
+    }
+  }
+}
 
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void k_gemm(const ChainDesc* chains, MatRef Cr, MatRef Ar, MatRef Br, VecRef dlr,
@@ -102,7 +193,7 @@ __global__ __launch_bounds__(256) void k_gemm(const ChainDesc* chains, MatRef Cr
   const int col = j0 + wc * 16 + (lane & 15);
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int row = i0 + wr * 16 + 4 * (lane >> 4) + v;
+    const int row = i0 + wr * 16 + kMfmaRow(lane, v);
     if (row < n && col < n) {
       double s = 1.0;
       if (dl) s *= dl[row];
@@ -115,9 +206,29 @@ __global__ __launch_bounds__(256) void k_gemm(const ChainDesc* chains, MatRef Cr
   }
 }
 
+static int use_vector_gemm() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("DQMC_GEMM_VECTOR");
+    v = (e && e[0] == '1') ? 1 : 0;
+  }
+  return v;
+}
+
 hipError_t launch_gemm(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef C, MatRef A,
                        GemmOp opA, MatRef B, GemmOp opB, VecRef dl, VecRef dr, double beta) {
   dim3 grid((nmax + GT - 1) / GT, (nmax + GT - 1) / GT, nchains);
+  if (use_vector_gemm()) {
+    if (opA == OP_N && opB == OP_N)
+      hipLaunchKernelGGL((k_gemm_vec<OP_N, OP_N>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
+    else if (opA == OP_N && opB == OP_C)
+      hipLaunchKernelGGL((k_gemm_vec<OP_N, OP_C>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
+    else if (opA == OP_C && opB == OP_N)
+      hipLaunchKernelGGL((k_gemm_vec<OP_C, OP_N>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
+    else
+      hipLaunchKernelGGL((k_gemm_vec<OP_C, OP_C>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
+    return hipGetLastError();
+  }
   if (opA == OP_N && opB == OP_N)
     hipLaunchKernelGGL((k_gemm<OP_N, OP_N>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
   else if (opA == OP_N && opB == OP_C)

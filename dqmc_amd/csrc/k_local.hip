@@ -325,15 +325,17 @@ hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int ncha
   const size_t lds = (size_t)(9 * Nmax + 2 * Nmax + 1) * 8 + (size_t)4 * nmax * 16 +
                      (size_t)4 * ((nmax + wg - 1) / wg + 1) * 16;
   if (lds > 150 * 1024) return hipErrorInvalidValue;
-  static bool attr = false;
-  if (!attr) {
+  static size_t attr = 0;
+  if (lds > 65536 && lds > attr) {
     hipError_t e =
-        hipFuncSetAttribute((const void*)k_local_updates, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipFuncSetAttribute((const void*)k_local_updates, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    attr = true;
+    attr = lds;
   }
   void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log};
-  return hipLaunchCooperativeKernel((const void*)k_local_updates, dim3(wg, nchains), dim3(256), args, lds, s);
+  hipError_t e = hipLaunchCooperativeKernel((const void*)k_local_updates, dim3(wg, nchains), dim3(256), args, lds, s);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
 }
 
 // ---------------------------------------------------------------------------

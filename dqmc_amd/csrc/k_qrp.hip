@@ -291,25 +291,35 @@ static hipError_t coop_launch(const void* fn, dim3 grid, void** args, hipStream_
   return hipLaunchCooperativeKernel(fn, grid, dim3(256), args, 0, s);
 }
 
+hipError_t launch_qrp(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D, int wg) {
+  if (nmax > kMaxN) return hipErrorInvalidValue;
+  void* args[] = {(void*)&chains, (void*)&A, (void*)&D};
+  hipError_t e = coop_launch((const void*)k_qrp, dim3(wg, nchains), args, s);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
+}
+
 hipError_t launch_udt(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
                       MatRef Q, MatRef T, int wg) {
+  hipError_t e = launch_qrp(s, chains, nchains, nmax, A, D, wg);
+  if (e != hipSuccess) return e;
+  return launch_form_TQ(s, chains, nchains, nmax, A, D, Q, T);
+}
+
+hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
+                          MatRef Q, MatRef T) {
   if (nmax > kMaxN) return hipErrorInvalidValue;
-  {
-    void* args[] = {(void*)&chains, (void*)&A, (void*)&D};
-    hipError_t e = coop_launch((const void*)k_qrp, dim3(wg, nchains), args, s);
-    if (e != hipSuccess) return e;
-  }
   const size_t nn = (size_t)nmax * nmax;
   int blocks = (int)((nn + 255) / 256);
   if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_form_T, dim3(blocks, nchains), dim3(256), 0, s, chains, A, D, T);
   const int cb = nmax <= 512 ? 8 : 4;
   const size_t lds = (size_t)2 * cb * nmax * sizeof(cplx);
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_form_Q, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  static size_t attr = 0;
+  if (lds > 65536 && lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_form_Q, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    attr = true;
+    attr = lds;
   }
   hipLaunchKernelGGL(k_form_Q, dim3((nmax + cb - 1) / cb, nchains), dim3(256), lds, s, chains, A, Q, cb);
   return hipGetLastError();
@@ -445,7 +455,10 @@ hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, i
   if ((nmax + wg - 1) / wg > kMaxN / 2) return hipErrorInvalidValue;
   void* args[] = {(void*)&chains, (void*)&X};
   hipError_t e = coop_launch((const void*)k_gj_inverse, dim3(wg, nchains), args, s);
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return e;
+  }
   hipLaunchKernelGGL(k_gj_unpivot, dim3((nmax + 255) / 256, nchains), dim3(256), 0, s, chains, X);
   return hipGetLastError();
 }

@@ -183,6 +183,25 @@ int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st);
 /* capture each shape group's sweep in a HIP graph and replay it (1) or launch eagerly (0) */
 int dqmc_batch_use_graph(dqmc_batch* b, int enable);
 
+/* ---- kernel timing -------------------------------------------------------
+ * No reference counterpart (the reference's TimerOutputs sections,
+ * dqmc_framework.jl:36-43, time the same phases on the CPU). When a class bit
+ * is set, every launch of that class is bracketed by HIP events on the chain
+ * group's stream; _read synchronises, returns accumulated milliseconds and
+ * launch counts per class, and clears the records. */
+#define DQMC_K_APPLY_B 0  /* fused slice-matrix application (checkerboard) */
+#define DQMC_K_GEMM 1     /* complex FP64 GEMM */
+#define DQMC_K_QRP 2      /* column-pivoted Householder QR */
+#define DQMC_K_QFORM 3    /* Q and T formation of the UDT */
+#define DQMC_K_INVERSE 4  /* Gauss-Jordan inverse */
+#define DQMC_K_LOCAL 5    /* local updates of one slice */
+#define DQMC_K_OTHER 6    /* copies and elementwise helpers */
+#define DQMC_K_NCLASS 7
+int dqmc_profile(dqmc_ctx* ctx, unsigned class_mask);
+int dqmc_profile_read(dqmc_ctx* ctx, double* ms, long long* count, int nclass);
+int dqmc_batch_profile(dqmc_batch* b, unsigned class_mask);
+int dqmc_batch_profile_read(dqmc_batch* b, double* ms, long long* count, int nclass);
+
 #ifdef __cplusplus
 }
 #endif
