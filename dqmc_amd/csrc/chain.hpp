@@ -29,7 +29,7 @@ enum MatId : int {
   kNumMat
 };
 // Named length-n real vectors.
-enum VecId : int { V_DL = 0, V_DR, V_D1, V_D2, V_DG, V_ONES, V_QRD, kNumVec };
+enum VecId : int { V_DL = 0, V_DR, V_D1, V_D2, V_D3, V_D4, V_DG, V_ONES, V_QRD, kNumVec };
 
 enum MatKind : int { MK_NAMED = 0, MK_USTACK = 1, MK_TSTACK = 2 };
 struct MatRef {
@@ -150,6 +150,9 @@ hipError_t launch_scale_rows_copy(hipStream_t s, const ChainDesc* chains, int nc
 hipError_t launch_scale_cols(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, VecRef d);
 hipError_t launch_add_diag(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, VecRef d);
 hipError_t launch_recip_vec(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef dst, VecRef src);
+// inv_big = 1 / max(D, 1), small = min(D, 1)
+hipError_t launch_split_scales(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef D,
+                               VecRef inv_big, VecRef small);
 hipError_t launch_adjoint(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef dst, MatRef src);
 // scal[S_DRIFT_LAST] = max|A - B|, scal[S_DRIFT_MAX] = max(that, old)
 hipError_t launch_maxdiff(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B);

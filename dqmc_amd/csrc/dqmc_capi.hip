@@ -65,6 +65,7 @@ struct Group {
   hipStream_t stream = nullptr;
   int cur = 0, dir = 1;  // reference current_slice (1-based), direction
   int wg_qr = 1, wg_inv = 1, wg_local = 1;
+  int greens_route = 0;  // 0: D-scale separation, 1: verbatim reference route
   bool recomputed = false;
   // kernel-class timing (dqmc_profile)
   unsigned prof_mask = 0;
@@ -442,6 +443,40 @@ struct Ops {
 
   // calculate_greens (stack.jl:338-369) [+ calculate_logdet (stack.jl:377-385)]
   int calculate_greens(bool logdet) {
+    return g.greens_route == 0 ? calculate_greens_scalesep() : calculate_greens_reference(logdet);
+  }
+
+  // G = [1 + Ul Dl Tl (Ur Dr Tr)^H]^-1 with the D scales separated into Db = max(D, 1)
+  // and Ds = min(D, 1):
+  //   G = Ur Drb^-1 [Dlb^-1 Ul^H Ur Drb^-1 + Dls Tl Tr^H Drs]^-1 Dlb^-1 Ul^H,
+  // the bracket being well conditioned, so one partial-pivoting inverse replaces the two
+  // inner UDT decompositions of the verbatim route; log|det G| = -log|det [.]| - sum log Dlb
+  // - sum log Drb (Ul, Ur unitary). Always leaves log|det G| in S_LOGDET.
+  int calculate_greens_scalesep() {
+    int rc;
+    LCHK(launch_split_scales(s(), d(), nc(), g.n, NV(V_DL), NV(V_D1), NV(V_D2)));  // 1/Dlb, Dls
+    LCHK(launch_split_scales(s(), d(), nc(), g.n, NV(V_DR), NV(V_D3), NV(V_D4)));  // 1/Drb, Drs
+    {
+      ProfScope ps(g, DQMC_K_GEMM);
+      LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_TMP), NM(M_UL), OP_C, NM(M_UR), OP_N, NV(V_D1), NV(V_D3), 0.0));
+      LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_TMP), NM(M_TL), OP_N, NM(M_TR), OP_C, NV(V_D2), NV(V_D4), 1.0));
+    }
+    if ((rc = inverse(NM(M_TMP)))) return rc;  // log|det [.]| -> S_LOGDET_A
+    LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D1)));
+    {
+      ProfScope ps(g, DQMC_K_GEMM);
+      LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_TMP), OP_N, NM(M_UL), OP_C, NV(V_D3), NOVEC, 0.0));
+      LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_G), NM(M_UR), OP_N, NM(M_TMP2), OP_N, NOVEC, NOVEC, 0.0));
+    }
+    LCHK(launch_sum_log(s(), d(), nc(), g.n, NV(V_D1), S_LOGDET, 0));
+    LCHK(launch_sum_log(s(), d(), nc(), g.n, NV(V_D3), S_LOGDET, 1));
+    LCHK(launch_negate_scalar(s(), d(), nc(), S_LOGDET_A));
+    LCHK(launch_add_scalar(s(), d(), nc(), S_LOGDET, S_LOGDET_A));
+    return 0;
+  }
+
+  // calculate_greens (stack.jl:338-369) verbatim: two inner UDT decompositions.
+  int calculate_greens_reference(bool logdet) {
     int rc;
     // tmp = diag(Dl) Tl Tr^H diag(Dr)
     if ((rc = gemm(NM(M_TMP), NM(M_TL), OP_N, NM(M_TR), OP_C, NV(V_DL), NV(V_DR)))) return rc;
@@ -788,6 +823,13 @@ int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local) {
   if (wg_qr > 0) g.wg_qr = wg_qr;
   if (wg_inv > 0) g.wg_inv = std::max(wg_inv, (g.n + 511) / 512);
   if (wg_local > 0) g.wg_local = wg_local;
+  return 0;
+}
+
+int dqmc_set_greens_route(dqmc_ctx* ctx, int route) {
+  CTX_CHECK(ctx);
+  if (route != 0 && route != 1) return fail(DQMC_E_ARG, "route must be 0 or 1");
+  ctx->b->groups[0].greens_route = route;
   return 0;
 }
 

@@ -342,6 +342,23 @@ hipError_t launch_ones_vec(hipStream_t s, const ChainDesc* chains, int nchains, 
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_split_scales(const ChainDesc* chains, VecRef Dr, VecRef Br, VecRef Sr) {
+  const ChainDesc& c = chains[blockIdx.y];
+  const double* D = vecptr(c, Dr);
+  double* ib = vecptr(c, Br);
+  double* sm = vecptr(c, Sr);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
+    const double v = D[i];
+    ib[i] = 1.0 / fmax(v, 1.0);
+    sm[i] = fmin(v, 1.0);
+  }
+}
+hipError_t launch_split_scales(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef D,
+                               VecRef inv_big, VecRef small) {
+  hipLaunchKernelGGL(k_split_scales, dim3((nmax + 255) / 256, nchains), dim3(256), 0, s, chains, D, inv_big, small);
+  return hipGetLastError();
+}
+
 // max|A - B| per chain: one block per chain, then fold into the drift slots.
 __global__ __launch_bounds__(256) void k_maxdiff(const ChainDesc* chains, MatRef Ar, MatRef Br) {
   const ChainDesc& c = chains[blockIdx.x];

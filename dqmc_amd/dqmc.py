@@ -185,6 +185,10 @@ class DQMC:
     def set_groups(self, wg_qr=0, wg_inv=0, wg_local=0):
         check(self.lib.dqmc_ctx_set_groups(self.h, wg_qr, wg_inv, wg_local))
 
+    def set_greens_route(self, route):
+        """0: D-scale separation (default); 1: the reference's calculate_greens verbatim."""
+        check(self.lib.dqmc_set_greens_route(self.h, route))
+
     # -- state ---------------------------------------------------------------
     @property
     def greens(self):

@@ -95,6 +95,12 @@ int dqmc_ctx_create(const dqmc_params* p, const dqmc_operators* ops, int device,
 int dqmc_ctx_destroy(dqmc_ctx* ctx);
 /* workgroups per chain for the cooperative kernels (QR, inverse, local updates); 0 = default */
 int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local);
+/* how calculate_greens (stack.jl:338-369) forms G = [1 + Ul Dl Tl (Ur Dr Tr)^H]^-1:
+ * 0 (default) separates the D scales into max(D,1) and min(D,1) and inverts the
+ * well-conditioned middle matrix once; 1 follows the reference verbatim (two
+ * inner UDT decompositions, myrdiv and inv). Both give the same G and log|det G|
+ * to rounding; the setting applies to every later recompute of the context. */
+int dqmc_set_greens_route(dqmc_ctx* ctx, int route);
 
 /* p.hsfield (parameters.jl:19): phi[a + 3*(i + N*l)] */
 int dqmc_set_field(dqmc_ctx* ctx, const double* phi);

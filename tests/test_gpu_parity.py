@@ -161,12 +161,14 @@ def test_slice_matrix_config_size(amd):
     c.close()
 
 
-def test_calculate_greens_kat(amd):
-    # tests_O3.jl:215-230
+@pytest.mark.parametrize("route", [0, 1])
+def test_calculate_greens_kat(amd, route):
+    # tests_O3.jl:215-230; route 0 = D-scale separation, 1 = the reference's two-UDT route
     mod, DQMC = amd
-    _say("calculate_greens KAT")
+    _say("calculate_greens KAT route %d" % route)
     p = fixture_params(mod)
     c = DQMC(p)
+    c.set_greens_route(route)
     conf = reconstruct_seeded_conf(ld)
     c.hsfield = conf
     G, ldet = c.calculate_greens(ld("Ul"), ld("Dl"), ld("Tl"), ld("Ur"), ld("Dr"), ld("Tr"))
@@ -180,13 +182,15 @@ def test_calculate_greens_kat(amd):
 
 
 # --------------------------------------------------------------------------- chain
-def test_propagation_full_cycle_fixture(amd):
+@pytest.mark.parametrize("route", [0, 1])
+def test_propagation_full_cycle_fixture(amd, route):
     # tests_O3.jl:240-263: every slice of a full up-down cycle within 1e-12 of the direct G
     mod, DQMC = amd
-    _say("propagation up-down cycle L=4 M=10 (bfield fixture model)")
+    _say("propagation up-down cycle L=4 M=10 (bfield fixture model), route %d" % route)
     p = fixture_params(mod)
     conf = reconstruct_seeded_conf(ld)
     c = DQMC(p)
+    c.set_greens_route(route)
     c.init(conf)
     assert (c.current_slice, c.direction) == (10, -1)
     o = O.DQMC(oracle_params(p), conf)
