@@ -77,6 +77,7 @@ SIGNATURES = {
     "dqmc_inverse": [_P, _D, _D, _D],
     "dqmc_boson_action": [_P, _D],
     "dqmc_grad": [_P, _D, _D],
+    "dqmc_hmc_trajectory": [_P, _D, _i, _d, _d, _I, _D],
     "dqmc_batch_create": [ctypes.POINTER(Params), ctypes.POINTER(Operators), _i, _i, ctypes.POINTER(_P)],
     "dqmc_batch_destroy": [_P],
     "dqmc_batch_set_field": [_P, _i, _D],

@@ -24,6 +24,7 @@ enum MatId : int {
   M_UG, M_TG,  // s.U and s.T kept after calculate_greens (logdet)
   M_IDENT,     // identity
   M_GRAD,      // spare n x n for the gradient/TDGF paths
+  M_INVT,      // output of the Gauss-Jordan column permutation
   M_ET2,       // dense exp(-dtau T/2)^2        (no checkerboard; operator, read-only)
   M_ETINV2,    // dense exp(+dtau T/2)^2
   kNumMat
@@ -97,6 +98,7 @@ enum ScalarId : int {
   S_LOGDET_A,          // log|det| scratch from inversions
   S_NONREAL,           // proposals with |Im r / Re r| > 1e-4
   S_MINRATIO,          // min |p_acc - u| over proposals that drew u
+  S_DRIFT_TMP,         // accumulator of the max-abs difference kernel
   kNumScalar = 16
 };
 
@@ -190,6 +192,7 @@ hipError_t launch_reset_stats(hipStream_t s, const ChainDesc* chains, int nchain
 hipError_t launch_boson_action(hipStream_t s, const ChainDesc* chains, int nchains, int Nmax);
 
 hipError_t launch_boson_grad(hipStream_t s, const ChainDesc* chains, int nchains, int NMmax, double* const* grad);
+hipError_t launch_daxpy(hipStream_t s, double* y, const double* x, double a, long long n);
 // dS/dphi fermion part at one slice from G (slice 0-based): grad[3][N] += -Re tr[(1-G)_ii eV^-1 d eV]
 hipError_t launch_grad_slice(hipStream_t s, const ChainDesc* chains, int nchains, int Nmax, int slice,
                              double* const* grad);

@@ -55,6 +55,11 @@ struct ChainMem {
   long long ucap = 0;
   double* log_buf = nullptr;
   long long log_cap = 0;
+  // gradient / HMC buffers (allocated on first use)
+  double* grad = nullptr;
+  double** grad_ptr = nullptr;
+  double* mom = nullptr;
+  double* phi_bak = nullptr;
 };
 
 struct Group {
@@ -255,8 +260,13 @@ void chain_free(ChainMem& cm) {
   cm.allocs.clear();
   if (cm.uniforms) (void)hipFree(cm.uniforms);
   if (cm.log_buf) (void)hipFree(cm.log_buf);
+  for (void* p : {(void*)cm.grad, (void*)cm.grad_ptr, (void*)cm.mom, (void*)cm.phi_bak})
+    if (p) (void)hipFree(p);
   cm.uniforms = nullptr;
   cm.log_buf = nullptr;
+  cm.grad = nullptr;
+  cm.grad_ptr = nullptr;
+  cm.mom = cm.phi_bak = nullptr;
 }
 
 int upload_desc(dqmc_batch* b, Group& g) {
@@ -1118,37 +1128,81 @@ int dqmc_boson_action(dqmc_ctx* ctx, double* S) {
   return 0;
 }
 
+// dS/dphi of the current field into the context's gradient buffer (enqueue only):
+// build_stack, then one down pass over all slices; at slice l the Green's function G(l)
+// of the stack (stack.jl:336) gives the fermionic part, the bosonic part is analytic.
+// Afterwards scal[S_BOSON_ACTION] = S_B and scal[S_LOGDET] = log|det G|.
+static int grad_enqueue(dqmc_batch* b) {
+  Group& g = b->groups[0];
+  ChainMem& cm = b->chains[0];
+  Ops o(g);
+  int rc;
+  if (!cm.grad) {
+    HIPCHK(hipMalloc(&cm.grad, sizeof(double) * 3 * cm.N * cm.M));
+    HIPCHK(hipMalloc(&cm.grad_ptr, sizeof(double*)));
+    HIPCHK(hipMemcpy(cm.grad_ptr, &cm.grad, sizeof(double*), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&cm.mom, sizeof(double) * 3 * cm.N * cm.M));
+    HIPCHK(hipMalloc(&cm.phi_bak, sizeof(double) * 3 * cm.N * cm.M));
+  }
+  if ((rc = o.build_stack())) return rc;
+  if ((rc = o.propagate())) return rc;  // -> slice M, direction -1, log det of G(1)
+  LCHK(launch_boson_grad(g.stream, g.d_desc, 1, cm.N * cm.M, cm.grad_ptr));
+  LCHK(launch_boson_action(g.stream, g.d_desc, 1, g.N));
+  for (int step = 0; step < g.M; ++step) {
+    LCHK(launch_grad_slice(g.stream, g.d_desc, 1, g.N, g.cur - 1, cm.grad_ptr));
+    if (step + 1 < g.M && (rc = o.propagate())) return rc;
+  }
+  return 0;
+}
+
 int dqmc_grad(dqmc_ctx* ctx, double* grad, double* S) {
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  ChainMem& cm = b->chains[0];
+  RC(grad_enqueue(b));
+  RC(batch_sync(b));
+  if (grad) HIPCHK(hipMemcpy(grad, cm.grad, sizeof(double) * 3 * cm.N * cm.M, hipMemcpyDeviceToHost));
+  if (S) *S = read_scalar(b, 0, S_BOSON_ACTION) + read_scalar(b, 0, S_LOGDET);
+  return 0;
+}
+
+int dqmc_hmc_trajectory(dqmc_ctx* ctx, const double* momenta, int nsteps, double eps, double u, int* accepted,
+                        double* dH) {
   CTX_CHECK(ctx);
   dqmc_batch* b = ctx->b;
   Group& g = b->groups[0];
   ChainMem& cm = b->chains[0];
-  Ops o(g);
-  double* dgrad = nullptr;
-  double** dptr = nullptr;
-  HIPCHK(hipMalloc(&dgrad, sizeof(double) * 3 * cm.N * cm.M));
-  HIPCHK(hipMalloc(&dptr, sizeof(double*)));
-  HIPCHK(hipMemcpy(dptr, &dgrad, sizeof(double*), hipMemcpyHostToDevice));
-  int rc = 0;
-  do {
-    if ((rc = o.build_stack())) break;
-    if ((rc = o.propagate())) break;  // -> slice M, direction -1, with log det
-    LCHK(launch_boson_grad(g.stream, g.d_desc, 1, cm.N * cm.M, dptr));
-    LCHK(launch_boson_action(g.stream, g.d_desc, 1, g.N));
-    // G(l) for l = M .. 1 on the down pass
-    for (int step = 0; step < g.M; ++step) {
-      LCHK(launch_grad_slice(g.stream, g.d_desc, 1, g.N, g.cur - 1, dptr));
-      if (step + 1 < g.M)
-        if ((rc = o.propagate())) break;
-    }
-    if (rc) break;
-    if ((rc = batch_sync(b))) break;
-    if (grad) HIPCHK(hipMemcpy(grad, dgrad, sizeof(double) * 3 * cm.N * cm.M, hipMemcpyDeviceToHost));
-    if (S) *S = read_scalar(b, 0, S_BOSON_ACTION) + read_scalar(b, 0, S_LOGDET);
-  } while (0);
-  (void)hipFree(dgrad);
-  (void)hipFree(dptr);
-  return rc;
+  if (!momenta || nsteps < 1 || !(eps > 0)) return fail(DQMC_E_ARG, "bad HMC arguments");
+  const long long nf = 3LL * cm.N * cm.M;
+  double k0 = 0.0;
+  for (long long e = 0; e < nf; ++e) k0 += 0.5 * momenta[e] * momenta[e];
+  RC(grad_enqueue(b));  // allocates the buffers on first use
+  HIPCHK(hipMemcpyAsync(cm.mom, momenta, sizeof(double) * nf, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(cm.phi_bak, cm.d.phi, sizeof(double) * nf, hipMemcpyDeviceToDevice, g.stream));
+  RC(batch_sync(b));
+  const double S0 = read_scalar(b, 0, S_BOSON_ACTION) + read_scalar(b, 0, S_LOGDET);
+  // leapfrog: p -= eps/2 dS; (phi += eps p; dS; p -= eps dS) ...; last kick eps/2
+  LCHK(launch_daxpy(g.stream, cm.mom, cm.grad, -0.5 * eps, nf));
+  for (int s = 0; s < nsteps; ++s) {
+    LCHK(launch_daxpy(g.stream, cm.d.phi, cm.mom, eps, nf));
+    RC(grad_enqueue(b));
+    LCHK(launch_daxpy(g.stream, cm.mom, cm.grad, (s + 1 < nsteps) ? -eps : -0.5 * eps, nf));
+  }
+  RC(batch_sync(b));
+  const double S1 = read_scalar(b, 0, S_BOSON_ACTION) + read_scalar(b, 0, S_LOGDET);
+  std::vector<double> p1(nf);
+  HIPCHK(hipMemcpy(p1.data(), cm.mom, sizeof(double) * nf, hipMemcpyDeviceToHost));
+  double k1 = 0.0;
+  for (long long e = 0; e < nf; ++e) k1 += 0.5 * p1[e] * p1[e];
+  const double dh = (S1 + k1) - (S0 + k0);
+  const int acc = (dh <= 0.0) || (u < std::exp(-dh));
+  if (!acc) HIPCHK(hipMemcpyAsync(cm.d.phi, cm.phi_bak, sizeof(double) * nf, hipMemcpyDeviceToDevice, g.stream));
+  // leave the chain as init! would: boson action, build_stack, propagate
+  RC(group_init(b, g));
+  RC(batch_sync(b));
+  if (accepted) *accepted = acc;
+  if (dH) *dH = dh;
+  return 0;
 }
 
 // ---- batches ---------------------------------------------------------------

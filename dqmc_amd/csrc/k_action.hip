@@ -148,6 +148,18 @@ __global__ __launch_bounds__(256) void k_grad_slice(const ChainDesc* chains, int
   }
 }
 
+// y += a x over n doubles (HMC leapfrog kicks and drifts)
+__global__ __launch_bounds__(256) void k_daxpy(double* y, const double* x, double a, long long n) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+    y[e] = fma(a, x[e], y[e]);
+}
+hipError_t launch_daxpy(hipStream_t s, double* y, const double* x, double a, long long n) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_daxpy, dim3((unsigned)blocks), dim3(256), 0, s, y, x, a, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_grad_slice(hipStream_t s, const ChainDesc* chains, int nchains, int Nmax, int slice,
                              double* const* grad) {
   hipLaunchKernelGGL(k_grad_slice, dim3((Nmax + 255) / 256, nchains), dim3(256), 0, s, chains, slice, grad);

@@ -360,13 +360,28 @@ hipError_t launch_split_scales(hipStream_t s, const ChainDesc* chains, int nchai
 }
 
 // max|A - B| per chain: one block per chain, then fold into the drift slots.
-__global__ __launch_bounds__(256) void k_maxdiff(const ChainDesc* chains, MatRef Ar, MatRef Br) {
-  const ChainDesc& c = chains[blockIdx.x];
+// phase 0: clear the accumulator; phase 1: every block folds its max|A - B| into it with a
+// 64-bit atomic max (non-negative doubles order like their bit patterns); phase 2: publish.
+__global__ __launch_bounds__(256) void k_maxdiff(const ChainDesc* chains, MatRef Ar, MatRef Br, int phase) {
+  const ChainDesc& c = chains[blockIdx.y];
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(c.scal + S_DRIFT_TMP);
+  if (phase == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *acc = 0ull;
+    return;
+  }
+  if (phase == 2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const double t = __longlong_as_double((long long)*acc);
+      c.scal[S_DRIFT_LAST] = t;
+      c.scal[S_DRIFT_MAX] = fmax(c.scal[S_DRIFT_MAX], t);
+    }
+    return;
+  }
   const size_t nn = (size_t)c.n * c.n;
   const cplx* A = matptr(c, Ar);
   const cplx* B = matptr(c, Br);
   double m = 0.0;
-  for (size_t e = threadIdx.x; e < nn; e += blockDim.x) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
     const cplx d = A[e] - B[e];
     m = fmax(m, sqrt(abs2(d)));
   }
@@ -376,13 +391,16 @@ __global__ __launch_bounds__(256) void k_maxdiff(const ChainDesc* chains, MatRef
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    c.scal[S_DRIFT_LAST] = t;
-    c.scal[S_DRIFT_MAX] = fmax(c.scal[S_DRIFT_MAX], t);
+    const double t = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(acc, (unsigned long long)__double_as_longlong(t));
   }
 }
-hipError_t launch_maxdiff(hipStream_t s, const ChainDesc* chains, int nchains, int, MatRef A, MatRef B) {
-  hipLaunchKernelGGL(k_maxdiff, dim3(nchains), dim3(256), 0, s, chains, A, B);
+hipError_t launch_maxdiff(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B) {
+  int blocks = (int)(((size_t)nmax * nmax + 255) / 256);
+  if (blocks > 256) blocks = 256;
+  hipLaunchKernelGGL(k_maxdiff, dim3(1, nchains), dim3(64), 0, s, chains, A, B, 0);
+  hipLaunchKernelGGL(k_maxdiff, dim3(blocks, nchains), dim3(256), 0, s, chains, A, B, 1);
+  hipLaunchKernelGGL(k_maxdiff, dim3(1, nchains), dim3(64), 0, s, chains, A, B, 2);
   return hipGetLastError();
 }
 

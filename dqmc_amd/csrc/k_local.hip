@@ -149,9 +149,11 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
   double* phe = phl + 3 * N;       // [3N] earlier slice
   double* pla = phe + 3 * N;       // [3N] later slice
   int* nbl = reinterpret_cast<int*>(pla + 3 * N);  // [4N]
-  cplx* Acol = reinterpret_cast<cplx*>(smem + 9 * N + 2 * N + (N & 1));  // [n][4]
-  cplx* Brow = Acol + 4 * (size_t)n;                                      // [4][ncol]
+  double* ul = smem + 9 * N + 2 * N;               // [4N] this slice's uniforms
+  cplx* Acol = reinterpret_cast<cplx*>(smem + 13 * N + 2 * N + (N & 1));  // [n][4]
+  cplx* Brow = Acol + 4 * (size_t)n;                                       // [4][ncol]
   __shared__ LocalShared sh;
+  __shared__ cplx gblk[16];
 
   const int le = (l == 0) ? M - 1 : l - 1;  // time_neighbors (lattice.jl:144-153)
   const int la = (l == M - 1) ? 0 : l + 1;
@@ -161,10 +163,14 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
     pla[e] = c.phi[(size_t)3 * N * la + e];
   }
   for (int e = tid; e < 4 * N; e += blockDim.x) nbl[e] = c.nb[e];
+  // at most 4 draws per proposal: this slice consumes no more than 4N uniforms
+  const long long cur0 = *c.ucursor;
+  const long long navail = min((long long)4 * N, c.n_uniforms - cur0);
+  for (long long e = tid; e < navail; e += blockDim.x) ul[e] = c.uniforms[cur0 + e];
   __syncthreads();
 
   const double lam_dtau = c.lam * c.dtau;
-  long long cur = *c.ucursor;
+  long long cur = cur0;
   long long logi = dolog ? *c.log_count : 0;
   double dS_acc = 0.0;
   int nacc = 0, nonreal = 0;
@@ -173,14 +179,16 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
   cplx* Gb[2] = {c.mat[M_G], c.mat[M_GALT]};
 
   for (int i = 0; i < N; ++i) {
+    // the 4x4 site block of the current G, one element per lane
+    if (tid < 16) gblk[tid] = Gb[buf][(size_t)(i + (tid & 3) * N) * n + (i + (tid >> 2) * N)];
+    __syncthreads();
     if (tid == 0) {
       sh.accepted = 0;
       sh.stop = 0;
-      if (cur + 4 > c.n_uniforms) {
+      if ((cur - cur0) + 4 > navail) {
         sh.stop = 1;
       } else {
-        const cplx* G = Gb[buf];
-        const double* u = c.uniforms + cur;
+        const double* u = ul + (cur - cur0);
         double old[3], nw[3];
         for (int a = 0; a < 3; ++a) {
           old[a] = phl[3 * i + a];
@@ -218,7 +226,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
         cplx Gm[16];
         for (int a = 0; a < 4; ++a)
           for (int b = 0; b < 4; ++b) {
-            const cplx gv = G[(size_t)(i + b * N) * n + (i + a * N)];
+            const cplx gv = gblk[a * 4 + b];
             Gm[a * 4 + b] = mk((a == b ? 1.0 : 0.0) - gv.re, -gv.im);  // I - G_ii
           }
         mul4(Dm, Gm, Mm);
@@ -322,7 +330,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
                                 const int* slices, double box, int wg, int log) {
   const int Nmax = nmax / 4;
-  const size_t lds = (size_t)(9 * Nmax + 2 * Nmax + 1) * 8 + (size_t)4 * nmax * 16 +
+  const size_t lds = (size_t)(13 * Nmax + 2 * Nmax + 1) * 8 + (size_t)4 * nmax * 16 +
                      (size_t)4 * ((nmax + wg - 1) / wg + 1) * 16;
   if (lds > 150 * 1024) return hipErrorInvalidValue;
   static size_t attr = 0;

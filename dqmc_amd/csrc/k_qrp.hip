@@ -46,8 +46,8 @@ __global__ __launch_bounds__(256) void k_qrp(const ChainDesc* chains, MatRef Ar,
   cplx* A = matptr(c, Ar);
   double* D = vecptr(c, Dr);
   GroupSync g = chain_group(c);
-  double* vn1 = c.qr_vn;
-  double* vn2 = c.qr_vn + n;
+  // partial column norms of the own columns, indexed by physical column (LDS)
+  __shared__ double vn1[kMaxN], vn2[kMaxN];
   const double tol3z = sqrt(0.5 * 2.220446049250313e-16);
 
   __shared__ int perm[kMaxN], lpos[kMaxN];
@@ -112,23 +112,40 @@ __global__ __launch_bounds__(256) void k_qrp(const ChainDesc* chains, MatRef Ar,
   for (int i = 0; i < n; ++i) {
     if (group_failed(g)) return;
     const int buf = i & 1;
-    // ---- pivot: first maximum of the partial norms over logical positions >= i
-    if (tid == 0) {
-      int bw = -1;
+    // ---- pivot: first maximum of the partial norms over logical positions >= i.
+    // Every candidate slot is read by its own thread (one memory round trip), then
+    // reduced in wave 0 by (norm descending, logical position ascending).
+    if (wave == 0) {
       double bn = -1.0;
-      int bp = n;
-      for (int q = 0; q < W; ++q) {
+      int bp = n, bw = -1;
+      for (int q = lane; q < W; q += 64) {
         const int pq = c.qr_cand_pos[buf * W + q];
-        if (pq < 0) continue;
         const double nq = c.qr_cand_norm[buf * W + q];
-        if (bw < 0 || nq > bn || (nq == bn && pq < bp)) {
-          bw = q;
+        if (pq >= 0 && (bw < 0 || nq > bn || (nq == bn && pq < bp))) {
           bn = nq;
           bp = pq;
+          bw = q;
         }
       }
-      s_win = bw;
-      s_pos = bp;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double on = __shfl_xor(bn, off, 64);
+        const int op = __shfl_xor(bp, off, 64);
+        const int ow = __shfl_xor(bw, off, 64);
+        if (ow >= 0 && (bw < 0 || on > bn || (on == bn && op < bp))) {
+          bn = on;
+          bp = op;
+          bw = ow;
+        }
+      }
+      if (lane == 0) {
+        s_win = bw;
+        s_pos = bp;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int bp = s_pos;
       const int pc = perm[bp], pi = perm[i];
       perm[i] = pc;
       perm[bp] = pi;
@@ -342,6 +359,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const ChainDesc* chains, Mat
   __shared__ int ri[256];
   __shared__ int s_p;
   __shared__ double s_logdet;
+  __shared__ int pivl[kMaxN], idxl[kMaxN];
   const int nown = (n - w + W - 1) / W;  // own columns: w, w+W, ...
 
   if (tid == 0) s_logdet = 0.0;
@@ -381,7 +399,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const ChainDesc* chains, Mat
     }
     if (tid == 0) {
       s_p = ri[0];
-      if (w == 0) c.inv_piv[k] = s_p;
+      pivl[k] = s_p;
     }
     __syncthreads();
     const int p = s_p;
@@ -431,22 +449,24 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const ChainDesc* chains, Mat
     group_sync(g);
   }
   if (w == 0 && tid == 0) c.scal[S_LOGDET_A] = s_logdet;
-}
-
-// undo the row interchanges as column interchanges, last first
-__global__ __launch_bounds__(256) void k_gj_unpivot(const ChainDesc* chains, MatRef Xr) {
-  const ChainDesc& c = chains[blockIdx.y];
-  const int n = c.n;
-  cplx* X = matptr(c, Xr);
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  for (int k = n - 1; k >= 0; --k) {
-    const int p = c.inv_piv[k];
-    if (p != k) {
-      const cplx a = X[(size_t)k * n + r];
-      X[(size_t)k * n + r] = X[(size_t)p * n + r];
-      X[(size_t)p * n + r] = a;
+  // Undo the row interchanges as column interchanges (last first). The composed
+  // permutation is built in LDS; every workgroup gathers its own output columns
+  // into M_INVT (all columns are visible after the last group barrier).
+  if (tid == 0) {
+    for (int j = 0; j < n; ++j) idxl[j] = j;
+    for (int k = n - 1; k >= 0; --k) {
+      const int p = pivl[k];
+      const int t = idxl[k];
+      idxl[k] = idxl[p];
+      idxl[p] = t;
     }
+  }
+  __syncthreads();
+  cplx* Y = c.mat[M_INVT];
+  for (int e = tid; e < nown * n; e += blockDim.x) {
+    const int jl = e / n, r = e - jl * n;
+    const int j = w + jl * W;
+    Y[(size_t)j * n + r] = X[(size_t)idxl[j] * n + r];
   }
 }
 
@@ -459,8 +479,7 @@ hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, i
     (void)hipGetLastError();
     return e;
   }
-  hipLaunchKernelGGL(k_gj_unpivot, dim3((nmax + 255) / 256, nchains), dim3(256), 0, s, chains, X);
-  return hipGetLastError();
+  return launch_copy(s, chains, nchains, nmax, X, MatRef{MK_NAMED, M_INVT});
 }
 
 }  // namespace dqmc

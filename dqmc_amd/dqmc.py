@@ -411,3 +411,17 @@ class Batch:
         st = _capi.Stats()
         check(self.lib.dqmc_batch_get_stats(self.h, chain, ctypes.byref(st)))
         return st
+
+
+def _hmc_trajectory(self, momenta, nsteps, eps, u):
+    """One HMC trajectory on the device; returns (accepted, dH)."""
+    mom = np.asfortranarray(momenta, dtype=np.float64)
+    if mom.shape != (3, self.p.N, self.p.slices):
+        raise ValueError("momenta must have shape (3, N, M)")
+    acc = ctypes.c_int()
+    dh = ctypes.c_double()
+    check(self.lib.dqmc_hmc_trajectory(self.h, dptr(mom), nsteps, eps, u, ctypes.byref(acc), ctypes.byref(dh)))
+    return bool(acc.value), dh.value
+
+
+DQMC.hmc_trajectory = _hmc_trajectory

@@ -170,6 +170,14 @@ int dqmc_boson_action(dqmc_ctx* ctx, double* S);
  * S(phi) = S_B(phi) - log det(1 + B_M...B_1), computed from the Green's
  * functions of one stabilised pass over all slices. Also returns S (optional). */
 int dqmc_grad(dqmc_ctx* ctx, double* grad, double* S);
+/* No reference counterpart (its only global move is a uniform shift,
+ * global_updates.jl:10-16): one Hamiltonian Monte Carlo trajectory of phi with
+ * H = sum p^2/2 + S(phi). `momenta` (3, N, M) are the initial momenta (host
+ * N(0,1) draws), `nsteps` leapfrog steps of size `eps` with dqmc_grad forces;
+ * accepted iff dH <= 0 or u < exp(-dH) with the host uniform `u`. On rejection
+ * phi is restored. The chain is left as after init! (stack rebuilt, slice M). */
+int dqmc_hmc_trajectory(dqmc_ctx* ctx, const double* momenta, int nsteps, double eps, double u, int* accepted,
+                        double* dH);
 
 /* ---- batches of independent chains -------------------------------------- */
 /* No reference counterpart: many chains (possibly of different L, M) advanced

@@ -950,3 +950,68 @@ class UniformStream:
 def host_uniforms(seed, count):
     """The host uniform stream used by tests and bench: numpy MT19937 doubles in [0,1)."""
     return np.random.Generator(np.random.MT19937(seed)).random(count)
+
+
+# ----------------------------------------------------------------------------
+# Adjoint dS/dphi (no counterpart in the reference; checked by finite differences)
+# ----------------------------------------------------------------------------
+
+
+def boson_action_grad(p, phi):
+    """dS_B/dphi from action.jl:1-52 differentiated by hand."""
+    L, M = p.L, p.slices
+    h = phi.reshape((3, L, L, M), order="F")
+    g = np.zeros_like(h)
+    if not p.edrun:
+        g += (2 * h - np.roll(h, -1, axis=3) - np.roll(h, 1, axis=3)) / (p.delta_tau * p.c ** 2)
+        for ax in (1, 2):
+            g += p.delta_tau * (2 * h - np.roll(h, -1, axis=ax) - np.roll(h, 1, axis=ax))
+        sq = np.sum(h * h, axis=0)[None]
+        g += p.delta_tau * (p.r * h + p.u * sq * h)
+    else:
+        g += p.delta_tau * p.r * h
+    return g.reshape((3, p.N, M), order="F")
+
+
+def _ev_op_grad(p, op):
+    """(eVop(op,-1), [d eVop(op,+1)/d op_a for a = 0, 1, 2])."""
+    op = np.asarray(op, dtype=np.float64)
+    nrm = math.sqrt(float(op @ op))
+    ld = p.lam * p.delta_tau
+    x = ld * nrm
+    sh = math.sinh(x) / nrm
+    C = math.cosh(x)
+    K = lambda S, R: np.array([[0, S, 0, R], [np.conj(S), 0, -R, 0], [0, -R, 0, np.conj(S)],  # noqa: E731
+                               [R, 0, S, 0]], dtype=np.complex128)
+    St, Rt = 1j * op[1] - op[0], -op[2]
+    dsh = (ld * C - sh) / (nrm * nrm)
+    grads = []
+    for a in range(3):
+        Sa = [-1.0, 1j, 0.0][a]
+        Ra = [0.0, 0.0, -1.0][a]
+        dE = ld * sh * op[a] * np.eye(4) + dsh * op[a] * K(St, Rt) + sh * K(Sa, Ra)
+        grads.append(dE)
+    return interaction_matrix_exp_op(p, op, -1.0), grads
+
+
+def action_grad_trace(m):
+    """dS/dphi of S = S_B - log det(1 + B_M ... B_1) by the trace formula
+    d log det / d phi_{a,i,l} = Re tr[(1 - G(l))_ii eV_i^-1 d_a eV_i], with G(l) the
+    Green's function at slice l (stack.jl:336) from calc_greens."""
+    p = m.p
+    N = p.N
+    g = boson_action_grad(p, m.phi)
+    for l in range(p.slices):
+        G = calc_greens(m, l + 1)
+        for i in range(N):
+            OG = np.eye(4) - G[i::N, i::N]
+            Einv, dEs = _ev_op_grad(p, m.phi[:, i, l])
+            for a in range(3):
+                g[a, i, l] -= float(np.real(np.trace(OG @ (Einv @ dEs[a]))))
+    return g
+
+
+def action(m, phi=None):
+    """S(phi) = S_B(phi) + log|det G| (log|det G| = -log det(1 + B_M...B_1))."""
+    mm = m if phi is None else Model(m.p, phi, m.ops)
+    return calc_boson_action(m.p, mm.phi) + calc_greens_and_logdet(mm, 1)[1]

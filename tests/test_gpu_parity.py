@@ -363,6 +363,9 @@ def test_gradient_finite_differences(amd):
         return O.calc_boson_action(op, ph) + O.calc_greens_and_logdet(m, 1)[1]
 
     assert abs(S - action(phi)) < 1e-9 * abs(S)
+    # every component against the CPU FP64 trace formula (oracle.action_grad_trace)
+    g_o = O.action_grad_trace(O.Model(op, phi))
+    assert np.abs(g - g_o).max() < 1e-6 * max(1.0, np.abs(g_o).max())
     h = 1e-6
     for _ in range(6):
         a, i, l = rng.integers(3), rng.integers(p.N), rng.integers(p.slices)
@@ -392,3 +395,35 @@ def test_batch_matches_single_chains(amd):
         assert np.abs(b.get_field(k) - c.hsfield).max() == 0
         c.close()
     b.close()
+
+
+def test_hmc_trajectory_matches_cpu_leapfrog(amd):
+    """configs[3]-style trajectory (10 leapfrog steps, eps 0.02) on a small chain vs a CPU
+    leapfrog driven by the oracle's trace-formula gradient."""
+    mod, DQMC = amd
+    _say("HMC trajectory L=4 M=20 vs CPU leapfrog")
+    p = config_params(mod, 4, 20)
+    rng = np.random.default_rng(61)
+    phi = rng.uniform(-1, 1, (3, p.N, p.slices))
+    mom = rng.standard_normal((3, p.N, p.slices))
+    c = DQMC(p)
+    c.init(phi)
+    acc, dh = c.hmc_trajectory(mom, 10, 0.02, 0.0)
+    assert acc and abs(dh) < 0.5
+    op = oracle_params(p)
+    m = O.Model(op, phi.copy())
+    pm = mom.copy()
+    H0 = O.action(m) + 0.5 * np.sum(pm * pm)
+    pm -= 0.01 * O.action_grad_trace(m)
+    for s in range(10):
+        m.phi += 0.02 * pm
+        pm -= (0.02 if s < 9 else 0.01) * O.action_grad_trace(m)
+    H1 = O.action(m) + 0.5 * np.sum(pm * pm)
+    assert np.abs(c.hsfield - m.phi).max() < 1e-9
+    assert abs(dh - (H1 - H0)) < 1e-7
+    # a rejected trajectory restores the field (u above every acceptance probability)
+    phi_before = c.hsfield
+    acc2, dh2 = c.hmc_trajectory(mom * 3.0, 10, 0.2, 2.0)
+    if not acc2:
+        assert np.array_equal(c.hsfield, phi_before)
+    c.close()

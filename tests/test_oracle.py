@@ -256,3 +256,23 @@ def test_free_fermion_closed_form(chkr):
     c2 = O.DQMC(p, phi)
     c2.init()
     assert abs(c2.log_det - ldet) < 1e-9 * abs(ldet)
+
+
+def test_gradient_trace_formula_vs_finite_differences():
+    # no reference counterpart: the CPU trace formula is pinned by central differences
+    p = O.Params(4, 10, hoppings=(1.0, 0.5, 0.5, 1.0), lam=1.0, r=1.0)
+    rng = np.random.default_rng(8)
+    phi = rng.uniform(-1, 1, (3, p.N, p.slices))
+    m = O.Model(p, phi)
+    g = O.action_grad_trace(m)
+    gb = O.boson_action_grad(p, phi)
+    h = 1e-5
+    for _ in range(8):
+        a, i, l = rng.integers(3), rng.integers(p.N), rng.integers(p.slices)
+        pp, pm = phi.copy(), phi.copy()
+        pp[a, i, l] += h
+        pm[a, i, l] -= h
+        fd = (O.action(m, pp) - O.action(m, pm)) / (2 * h)
+        fdb = (O.calc_boson_action(p, pp) - O.calc_boson_action(p, pm)) / (2 * h)
+        assert abs(g[a, i, l] - fd) < 1e-7 * max(1.0, abs(fd)), (a, i, l, g[a, i, l], fd)
+        assert abs(gb[a, i, l] - fdb) < 1e-7 * max(1.0, abs(fdb))
