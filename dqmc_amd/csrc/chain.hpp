@@ -47,6 +47,7 @@ struct ChainDesc {
   // model
   int L, N, n, M, safe_mult, nelem, chkr;  // chkr: 1 = Assaad checkerboard, 0 = dense hopping
   int edrun;
+  int plaq_uniform;  // 1: every plaquette of a (variant, group, block) has the same 4x4 block
   double dtau, lam, r, c, u;
   // operators (host-computed at setup, read-only)
   const int* plaq_sites;  // [2 groups][N/4][4] site indices

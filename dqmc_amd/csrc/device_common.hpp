@@ -111,16 +111,19 @@ __device__ inline void group_sync(const GroupSync& g) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned int my_gen = __hip_atomic_load(g.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned int arrived =
+    // Read the generation before arriving; the arrival is a release (publishes this
+    // workgroup's writes). Polling uses relaxed loads so the wait does not invalidate
+    // caches on every iteration; one acquire fence after the wait makes the other
+    // workgroups' writes visible.
+    const unsigned int my_gen = __hip_atomic_load(g.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int arrived =
         __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
     if (arrived == g.nblocks) {
       __hip_atomic_store(g.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(g.gen, my_gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       long long spins = 0;
-      while (__hip_atomic_load(g.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
+      while (__hip_atomic_load(g.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
         if (__hip_atomic_load(g.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (++spins > kSpinLimit) {
           __hip_atomic_store(g.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -129,7 +132,7 @@ __device__ inline void group_sync(const GroupSync& g) {
         __builtin_amdgcn_s_sleep(1);
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }

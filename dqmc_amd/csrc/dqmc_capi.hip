@@ -244,6 +244,17 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
     d.plaq = pl;
     d.expmu = em;
     d.expmu_inv = emi;
+    // without a magnetic field every plaquette of a (variant, group, block) carries the same
+    // 4x4 block; the kernel then stages just those blocks in LDS
+    {
+      const int np = N / 4;
+      const double* P = o->plaq;
+      bool uni = true;
+      for (int vgb = 0; vgb < 4 * 2 * 4 && uni; ++vgb)
+        for (int q = 1; q < np && uni; ++q)
+          uni = memcmp(P + 32 * ((size_t)vgb * np + q), P + 32 * ((size_t)vgb * np), 32 * sizeof(double)) == 0;
+      d.plaq_uniform = uni ? 1 : 0;
+    }
   } else {
     // B = eT (eT eV): the product of the two half-step exponentials is formed once
     std::vector<double> e2 = host_cmatmul(o->hop_exp, o->hop_exp, n, false);

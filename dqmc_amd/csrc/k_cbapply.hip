@@ -79,6 +79,18 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
   cplx* vec = reinterpret_cast<cplx*>(smem);          // [vpb][n]
   double* evc = smem + 2 * (size_t)vpb * n;           // [N][4] C, S.re, S.im, R
 
+  // plaquette blocks shared by all plaquettes of a (stage, flavour block): one LDS copy
+  __shared__ cplx pl_lds[5][4][16];
+  const bool uni = c.plaq_uniform != 0;
+  const int nplaq_all = N / 4;
+  if (uni) {
+    for (int e = threadIdx.x; e < sl.count * 64; e += blockDim.x) {
+      const int s = e >> 6, b = (e >> 4) & 3, k = e & 15;
+      const Stage st = sl.st[s];
+      if (st.kind == ST_PLAQ)
+        pl_lds[s][b][k] = c.plaq[((((size_t)st.var * 2 + st.group) * 4 + b) * nplaq_all) * 16 + k];
+    }
+  }
   bool need_ev = false;
   for (int s = 0; s < sl.count; ++s) need_ev |= (sl.st[s].kind == ST_EV);
   if (need_ev) {
@@ -136,7 +148,7 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
         const int v = idx / (4 * nplaq);
         const int rem = idx - v * 4 * nplaq;
         const int b = rem / nplaq, pq = rem - b * nplaq;
-        const cplx* P = P0 + ((size_t)b * nplaq + pq) * 16;
+        const cplx* P = uni ? pl_lds[s][b] : P0 + ((size_t)b * nplaq + pq) * 16;
         const int* st4 = sites + 4 * pq;
         cplx* base = vec + (size_t)v * n + b * N;
         cplx x[4], y[4];
