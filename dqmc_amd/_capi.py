@@ -76,6 +76,13 @@ SIGNATURES = {
     "dqmc_gemm": [_P, _i, _i, _D, _D, _D],
     "dqmc_inverse": [_P, _D, _D, _D],
     "dqmc_boson_action": [_P, _D],
+    "dqmc_effective_greens2greens": [_P, _D],
+    "dqmc_tdgf_alloc": [_P],
+    "dqmc_tdgf_free": [_P],
+    "dqmc_measure_tdgfs": [_P],
+    "dqmc_get_tdgf": [_P, _i, _D, _D],
+    "dqmc_get_tdgf_stack": [_P, _i, _i, _D, _D, _D],
+    "dqmc_tdgf_direct": [_P, _i, _D],
     "dqmc_grad": [_P, _D, _D],
     "dqmc_hmc_trajectory": [_P, _D, _i, _d, _d, _I, _D],
     "dqmc_batch_create": [ctypes.POINTER(Params), ctypes.POINTER(Operators), _i, _i, ctypes.POINTER(_P)],

@@ -27,17 +27,20 @@ enum MatId : int {
   M_INVT,      // output of the Gauss-Jordan column permutation
   M_ET2,       // dense exp(-dtau T/2)^2        (no checkerboard; operator, read-only)
   M_ETINV2,    // dense exp(+dtau T/2)^2
+  M_ET,        // dense exp(-dtau T/2)          (no checkerboard; effective -> true G)
+  M_ETINV,     // dense exp(+dtau T/2)
   kNumMat
 };
 // Named length-n real vectors.
-enum VecId : int { V_DL = 0, V_DR, V_D1, V_D2, V_D3, V_D4, V_DG, V_ONES, V_QRD, kNumVec };
+enum VecId : int { V_DL = 0, V_DR, V_D1, V_D2, V_D3, V_D4, V_D5, V_D6, V_DG, V_ONES, V_QRD, kNumVec };
 
-enum MatKind : int { MK_NAMED = 0, MK_USTACK = 1, MK_TSTACK = 2 };
+// MK_XMAT / VK_XVEC address the time-displaced Green's function storage (ChainDesc::xmat)
+enum MatKind : int { MK_NAMED = 0, MK_USTACK = 1, MK_TSTACK = 2, MK_XMAT = 3 };
 struct MatRef {
   int kind;
   int idx;
 };
-enum VecKind : int { VK_NAMED = 0, VK_DSTACK = 1 };
+enum VecKind : int { VK_NAMED = 0, VK_DSTACK = 1, VK_XVEC = 2 };
 struct VecRef {
   int kind;
   int idx;
@@ -62,6 +65,10 @@ struct ChainDesc {
   cplx* ustack;           // [nelem][n*n]
   cplx* tstack;           // [nelem][n*n]
   double* dstack;         // [nelem][n]
+  // time-displaced Green's functions (allocated on request): xmat holds the four
+  // checkpoint chains' U and T, then Gt0[1..M], then G0t[1..M]; xvec the chains' D
+  cplx* xmat;
+  double* xvec;
   // factorization scratch
   double* qr_vn;          // [2n] partial column norms
   cplx* qr_cand;          // [2][W][n] candidate columns (double buffered)
@@ -107,10 +114,12 @@ __device__ inline cplx* matptr(const ChainDesc& c, MatRef r) {
   const size_t nn = (size_t)c.n * c.n;
   if (r.kind == MK_NAMED) return c.mat[r.idx];
   if (r.kind == MK_USTACK) return c.ustack + nn * r.idx;
+  if (r.kind == MK_XMAT) return c.xmat + nn * r.idx;
   return c.tstack + nn * r.idx;
 }
 __device__ inline double* vecptr(const ChainDesc& c, VecRef r) {
   if (r.kind == VK_NAMED) return c.vec[r.idx];
+  if (r.kind == VK_XVEC) return c.xvec + (size_t)c.n * r.idx;
   return c.dstack + (size_t)c.n * r.idx;
 }
 
@@ -127,7 +136,10 @@ __device__ inline GroupSync chain_group(const ChainDesc& c) {
 // ---------------------------------------------------------------------------
 namespace dqmc {
 
-enum BOp : int { B_LEFT = 0, B_RIGHT, B_INV_LEFT, B_INV_RIGHT, B_DAG_LEFT };
+enum BOp : int { B_LEFT = 0, B_RIGHT, B_INV_LEFT, B_INV_RIGHT, B_DAG_LEFT,
+                 // checkerboard halves of effective_greens2greens!: X hop_half[B] hop_half[A] and
+                 // hop_half_inv[A] hop_half_inv[B] X (no slice dependence)
+                 B_EFF_RIGHT, B_EFF_INV_LEFT };
 
 // X <- op(B_slice) X  or  X op(B_slice) with the checkerboard operators; slice is 0-based
 // and the same for all chains, or taken per chain from `slices` when non-null.
@@ -157,6 +169,11 @@ hipError_t launch_recip_vec(hipStream_t s, const ChainDesc* chains, int nchains,
 hipError_t launch_split_scales(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef D,
                                VecRef inv_big, VecRef small);
 hipError_t launch_adjoint(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef dst, MatRef src);
+// X += src diag(d)
+hipError_t launch_add_scaled_cols(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X,
+                                  MatRef src, VecRef d);
+// X = -X
+hipError_t launch_negate(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X);
 // scal[S_DRIFT_LAST] = max|A - B|, scal[S_DRIFT_MAX] = max(that, old)
 hipError_t launch_maxdiff(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B);
 

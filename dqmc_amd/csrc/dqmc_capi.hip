@@ -60,6 +60,9 @@ struct ChainMem {
   double** grad_ptr = nullptr;
   double* mom = nullptr;
   double* phi_bak = nullptr;
+  // time-displaced Green's function storage (dqmc_tdgf_alloc)
+  cplx* tdgf_mat = nullptr;
+  double* tdgf_vec = nullptr;
 };
 
 struct Group {
@@ -185,7 +188,7 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
     field = (type*)ptr;                                            \
   } while (0)
   for (int m = 0; m < kNumMat; ++m) {
-    if ((m == M_ET2 || m == M_ETINV2) && p->checkerboard) continue;
+    if ((m == M_ET2 || m == M_ETINV2 || m == M_ET || m == M_ETINV) && p->checkerboard) continue;
     ALLOC(d.mat[m], cplx, nn);
   }
   for (int v = 0; v < kNumVec; ++v) ALLOC(d.vec[v], double, n);
@@ -261,6 +264,8 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
     std::vector<double> ei2 = host_cmatmul(o->hop_exp_inv, o->hop_exp_inv, n, false);
     HIPCHK(hipMemcpy(d.mat[M_ET2], e2.data(), sizeof(cplx) * nn, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d.mat[M_ETINV2], ei2.data(), sizeof(cplx) * nn, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.mat[M_ET], o->hop_exp, sizeof(cplx) * nn, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d.mat[M_ETINV], o->hop_exp_inv, sizeof(cplx) * nn, hipMemcpyHostToDevice));
   }
 #undef ALLOC
   return 0;
@@ -271,8 +276,11 @@ void chain_free(ChainMem& cm) {
   cm.allocs.clear();
   if (cm.uniforms) (void)hipFree(cm.uniforms);
   if (cm.log_buf) (void)hipFree(cm.log_buf);
-  for (void* p : {(void*)cm.grad, (void*)cm.grad_ptr, (void*)cm.mom, (void*)cm.phi_bak})
+  for (void* p : {(void*)cm.grad, (void*)cm.grad_ptr, (void*)cm.mom, (void*)cm.phi_bak, (void*)cm.tdgf_mat,
+                  (void*)cm.tdgf_vec})
     if (p) (void)hipFree(p);
+  cm.tdgf_mat = nullptr;
+  cm.tdgf_vec = nullptr;
   cm.uniforms = nullptr;
   cm.log_buf = nullptr;
   cm.grad = nullptr;
@@ -611,6 +619,200 @@ struct Ops {
     return 0;
   }
 
+  // calc_Bchain (op B_LEFT), calc_Bchain_inv (B_INV_LEFT), calc_Bchain_dagger (B_DAG_LEFT)
+  // (fermion_measurements.jl:964-1058): U D T of the product over slices start..stop (1-based,
+  // inclusive), decomposed at every multiple of safe_mult and at the last slice taken.
+  int bchain(int start, int stop, BOp op, MatRef U, VecRef D, MatRef T) {
+    int rc;
+    LCHK(launch_identity(s(), d(), nc(), g.n, U, NOVEC));
+    LCHK(launch_identity(s(), d(), nc(), g.n, T, NOVEC));
+    LCHK(launch_ones_vec(s(), d(), nc(), g.n, D));
+    const bool down = op != B_LEFT;
+    for (int t = 0; t <= stop - start; ++t) {
+      const int k = down ? stop - t : start + t;
+      if ((rc = applyB(U, op, k - 1))) return rc;
+      if (k % g.s == 0 || k == (down ? start : stop)) {
+        LCHK(launch_scale_cols(s(), d(), nc(), g.n, U, D));
+        if ((rc = udt(U, D, NM(M_QRQ), NM(M_QRT)))) return rc;
+        if ((rc = copy(U, NM(M_QRQ)))) return rc;
+        if ((rc = gemm(NM(M_TMP), NM(M_QRT), OP_N, T, OP_N))) return rc;
+        if ((rc = copy(T, NM(M_TMP)))) return rc;
+      }
+    }
+    return 0;
+  }
+
+  // ---- time-displaced Green's functions (fermion_measurements.jl:1239-1541) ----------------
+  // Storage in ChainDesc::xmat, nr = M / safe_mult checkpoints per chain q (0 BT0Inv, 1 BBetaT,
+  // 2 BT0, 3 BBetaTInv): U of entry k at 2 (q nr + k), T at 2 (q nr + k) + 1, D at xvec q nr + k;
+  // then Gt0[tau] at 8 nr + tau - 1 and G0t[tau] at 8 nr + M + tau - 1 (tau = 1..M).
+  int nr() const { return g.M / g.s; }
+  MatRef XU(int q, int k) const { return MatRef{MK_XMAT, 2 * (q * nr() + k)}; }
+  MatRef XT(int q, int k) const { return MatRef{MK_XMAT, 2 * (q * nr() + k) + 1}; }
+  VecRef XD(int q, int k) const { return VecRef{VK_XVEC, q * nr() + k}; }
+  MatRef GT0(int tau) const { return MatRef{MK_XMAT, 8 * nr() + tau - 1}; }
+  MatRef G0T(int tau) const { return MatRef{MK_XMAT, 8 * nr() + g.M + tau - 1}; }
+
+  // effective_greens2greens! (fermion_measurements.jl:1125-1141, 1178-1185):
+  // X <- hop_half_inv[A] hop_half_inv[B] X hop_half[B] hop_half[A], or eT^-1 X eT. X != M_TMP4.
+  int effective_greens(MatRef X) {
+    if (g.chkr) {
+      ProfScope ps(g, DQMC_K_APPLY_B);
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, B_EFF_RIGHT, 0, nullptr));
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, B_EFF_INV_LEFT, 0, nullptr));
+      return 0;
+    }
+    int rc;
+    if ((rc = gemm(NM(M_TMP4), X, OP_N, NM(M_ET), OP_N))) return rc;
+    return gemm(X, NM(M_ETINV), OP_N, NM(M_TMP4), OP_N);
+  }
+  // the U, T form (fermion_measurements.jl:1143-1159, 1187-1193): T <- T hop_half[B] hop_half[A],
+  // U <- hop_half_inv[A] hop_half_inv[B] U (eT / eT^-1 without the checkerboard)
+  int effective_greens_ut(MatRef U, MatRef T) {
+    if (g.chkr) {
+      ProfScope ps(g, DQMC_K_APPLY_B);
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, T, B_EFF_RIGHT, 0, nullptr));
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, U, B_EFF_INV_LEFT, 0, nullptr));
+      return 0;
+    }
+    int rc;
+    if ((rc = gemm(NM(M_TMP4), T, OP_N, NM(M_ET), OP_N))) return rc;
+    if ((rc = copy(T, NM(M_TMP4)))) return rc;
+    if ((rc = gemm(NM(M_TMP4), NM(M_ETINV), OP_N, U, OP_N))) return rc;
+    return copy(U, NM(M_TMP4));
+  }
+
+  // calc_Bchain_udts! (fermion_measurements.jl:1425-1507) into chain q. Entries are indexed as
+  // after the reference's final reverse!: entry k covers slices 1 .. (k+1)s (left chains) or
+  // k s + 1 .. M (right chains).
+  int bchain_udts(int q, bool invert, bool right) {
+    int rc;
+    const bool rightmult = (right && !invert) || (!right && invert);
+    const int R = nr();
+    const BOp op = !invert ? (right ? B_RIGHT : B_LEFT) : (right ? B_INV_LEFT : B_INV_RIGHT);
+    const MatRef cur = NM(M_CURU);
+    for (int i = 0; i < R; ++i) {
+      const int k = right ? R - 1 - i : i;   // entry written by this step
+      const int kp = right ? k + 1 : k - 1;  // entry written by the previous step
+      if (i == 0)
+        LCHK(launch_identity(s(), d(), nc(), g.n, cur, NOVEC));
+      else if ((rc = copy(cur, rightmult ? XT(q, kp) : XU(q, kp))))
+        return rc;
+      for (int j = 0; j < g.s; ++j) {
+        const int sl = right ? (k + 1) * g.s - 1 - j : k * g.s + j;  // 0-based slice
+        if ((rc = applyB(cur, op, sl))) return rc;
+      }
+      if (i != 0) {
+        if (!rightmult)
+          LCHK(launch_scale_cols(s(), d(), nc(), g.n, cur, XD(q, kp)));
+        else
+          LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, cur, cur, XD(q, kp)));
+      }
+      if (!rightmult) {
+        if ((rc = udt(cur, XD(q, k), XU(q, k), i == 0 ? XT(q, k) : NM(M_QRT)))) return rc;
+        if (i != 0 && (rc = gemm(XT(q, k), NM(M_QRT), OP_N, XT(q, kp), OP_N))) return rc;
+      } else {
+        if ((rc = udt(cur, XD(q, k), i == 0 ? XU(q, k) : NM(M_QRQ), XT(q, k)))) return rc;
+        if (i != 0 && (rc = gemm(XU(q, k), XU(q, kp), OP_N, NM(M_QRQ), OP_N))) return rc;
+      }
+    }
+    return 0;
+  }
+
+  // res = [1 + U D T]^-1 (inv_one_plus_udt_scalettar!, linalg.jl:310-339); T^-1 stays in M_TMP
+  int inv_one_plus_udt_scalettar(MatRef res, MatRef U, VecRef D, MatRef T) {
+    int rc;
+    LCHK(launch_split_scales(s(), d(), nc(), g.n, D, NV(V_D1), NV(V_D2)));  // 1/Dp, Dm
+    if ((rc = copy(NM(M_TMP), T))) return rc;
+    if ((rc = inverse(NM(M_TMP)))) return rc;
+    if ((rc = copy(NM(M_TMP2), NM(M_TMP)))) return rc;
+    LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_TMP2), NV(V_D1)));            // l = T \ Dp^-1
+    LCHK(launch_add_scaled_cols(s(), d(), nc(), g.n, NM(M_TMP2), U, NV(V_D2)));    // r = U Dm + l
+    if ((rc = udt(NM(M_TMP2), NV(V_D3), NM(M_QRQ), NM(M_QRT)))) return rc;         // u, d, t
+    if ((rc = inverse(NM(M_QRT)))) return rc;
+    LCHK(launch_recip_vec(s(), d(), nc(), g.n, NV(V_D4), NV(V_D3)));
+    LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_QRT), NV(V_D4)));             // r = t \ d^-1
+    if ((rc = gemm(NM(M_TMP3), NM(M_QRT), OP_N, NM(M_QRQ), OP_C, NV(V_D1)))) return rc;  // Dp^-1 r u'
+    if ((rc = udt(NM(M_TMP3), NV(V_D3), NM(M_QRQ), NM(M_QRT)))) return rc;
+    if ((rc = gemm(NM(M_TMP2), NM(M_TMP), OP_N, NM(M_QRQ), OP_N, NOVEC, NV(V_D3)))) return rc;  // T \ u d
+    return gemm(res, NM(M_TMP2), OP_N, NM(M_QRT), OP_N);
+  }
+
+  // res = [Ua Da Ta + Ub Db Tb]^-1 (inv_sum_udts_scalettar!, linalg.jl:512-567 and its
+  // allocating form :442-496); Ta Tb^-1 and Tb \ U use one explicit Tb^-1.
+  int inv_sum_udts_scalettar(MatRef res, MatRef Ua, VecRef Da, MatRef Ta, MatRef Ub, VecRef Db, MatRef Tb,
+                             MatRef Uout = MatRef{-1, 0}, VecRef Dout = NOVEC, MatRef Tout = MatRef{-1, 0}) {
+    int rc;
+    LCHK(launch_split_scales(s(), d(), nc(), g.n, Da, NV(V_D1), NV(V_D2)));  // 1/Dap, Dam
+    LCHK(launch_split_scales(s(), d(), nc(), g.n, Db, NV(V_D3), NV(V_D4)));  // 1/Dbp, Dbm
+    if ((rc = copy(NM(M_TMP), Tb))) return rc;
+    if ((rc = inverse(NM(M_TMP)))) return rc;  // Tb^-1
+    if ((rc = gemm(NM(M_TMP2), Ta, OP_N, NM(M_TMP), OP_N, NV(V_D2), NV(V_D3)))) return rc;
+    {
+      ProfScope ps(g, DQMC_K_GEMM);
+      LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_TMP2), Ua, OP_C, Ub, OP_N, NV(V_D1), NV(V_D4), 1.0));
+    }
+    if ((rc = udt(NM(M_TMP2), NV(V_D5), NM(M_QRQ), NM(M_QRT)))) return rc;
+    LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, NM(M_QRT), NM(M_QRT), NV(V_D5)));  // D T
+    if ((rc = inverse(NM(M_QRT)))) return rc;
+    if ((rc = gemm(NM(M_TMP3), NM(M_QRT), OP_N, NM(M_QRQ), OP_C, NV(V_D3), NV(V_D1)))) return rc;
+    if ((rc = udt(NM(M_TMP3), NV(V_D5), NM(M_QRQ), NM(M_QRT)))) return rc;
+    if (Uout.kind >= 0) {
+      // U, D, T form: U = Tb^-1 U, T = T Ua'
+      if ((rc = gemm(Uout, NM(M_TMP), OP_N, NM(M_QRQ), OP_N))) return rc;
+      if ((rc = gemm(Tout, NM(M_QRT), OP_N, Ua, OP_C))) return rc;
+      return copyv(Dout, NV(V_D5));
+    }
+    if ((rc = gemm(NM(M_TMP2), NM(M_TMP), OP_N, NM(M_QRQ), OP_N, NOVEC, NV(V_D5)))) return rc;
+    if ((rc = gemm(NM(M_TMP3), NM(M_QRT), OP_N, Ua, OP_C))) return rc;
+    return gemm(res, NM(M_TMP2), OP_N, NM(M_TMP3), OP_N);
+  }
+
+  // measure_tdgfs! (fermion_measurements.jl:1343-1407) with fill_tdgf! (:1513-1541).
+  // G0t is negated at the checkpoint slices instead of after the fill; the fill is linear and
+  // IEEE negation commutes exactly with it, so the values are the same.
+  int measure_tdgfs() {
+    int rc;
+    if ((rc = bchain_udts(0, true, false))) return rc;   // BT0Inv
+    if ((rc = bchain_udts(1, false, true))) return rc;   // BBetaT
+    if ((rc = bchain_udts(2, false, false))) return rc;  // BT0
+    if ((rc = bchain_udts(3, true, true))) return rc;    // BBetaTInv
+    const int R = nr(), M = g.M, sm = g.s;
+    for (int i = 0; i < R; ++i) {
+      const int tau = i * sm + 1;
+      if (i != 0) {
+        if ((rc = inv_sum_udts_scalettar(GT0(tau), XU(0, i - 1), XD(0, i - 1), XT(0, i - 1), XU(1, i), XD(1, i),
+                                         XT(1, i))))
+          return rc;
+        if ((rc = inv_sum_udts_scalettar(G0T(tau), XU(2, i - 1), XD(2, i - 1), XT(2, i - 1), XU(3, i), XD(3, i),
+                                         XT(3, i))))
+          return rc;
+      } else {
+        if ((rc = inv_one_plus_udt_scalettar(GT0(1), XU(1, 0), XD(1, 0), XT(1, 0)))) return rc;
+        if ((rc = inv_one_plus_udt_scalettar(G0T(1), XU(3, 0), XD(3, 0), XT(3, 0)))) return rc;
+      }
+      if ((rc = effective_greens(GT0(tau)))) return rc;
+      if ((rc = effective_greens(G0T(tau)))) return rc;
+      LCHK(launch_negate(s(), d(), nc(), g.n, G0T(tau)));
+    }
+    const int mhalf = M / 2 + 1;
+    for (int tau = mhalf; tau <= M; ++tau) {
+      if ((tau - 1) % sm == 0) continue;
+      if ((rc = copy(GT0(tau), GT0(tau - 1)))) return rc;
+      if ((rc = applyB(GT0(tau), B_LEFT, tau - 1))) return rc;
+      if ((rc = copy(G0T(tau), G0T(tau - 1)))) return rc;
+      if ((rc = applyB(G0T(tau), B_INV_RIGHT, tau - 1))) return rc;
+    }
+    for (int tau = mhalf - 1; tau >= 1; --tau) {
+      if ((tau - 1) % sm == 0) continue;
+      if ((rc = copy(GT0(tau), GT0(tau + 1)))) return rc;
+      if ((rc = applyB(GT0(tau), B_INV_LEFT, tau))) return rc;  // B(tau + 1)^-1
+      if ((rc = copy(G0T(tau), G0T(tau + 1)))) return rc;
+      if ((rc = applyB(G0T(tau), B_RIGHT, tau))) return rc;
+    }
+    return 0;
+  }
+
   int local_updates(double box, int dolog) {
     if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "local_updates outside slices 1..M");
     ProfScope ps(g, DQMC_K_LOCAL);
@@ -769,32 +971,48 @@ int group_sweeps(Group& g, int nsweeps, double box) {
   return 0;
 }
 
+// device addresses of a chain's matrices and vectors, as matptr/vecptr compute them on the device
+cplx* host_matptr(const ChainMem& cm, MatRef r) {
+  const size_t nn = (size_t)cm.n * cm.n;
+  switch (r.kind) {
+    case MK_NAMED: return cm.d.mat[r.idx];
+    case MK_USTACK: return cm.d.ustack + nn * r.idx;
+    case MK_TSTACK: return cm.d.tstack + nn * r.idx;
+    default: return cm.d.xmat + nn * r.idx;
+  }
+}
+double* host_vecptr(const ChainMem& cm, VecRef r) {
+  switch (r.kind) {
+    case VK_NAMED: return cm.d.vec[r.idx];
+    case VK_DSTACK: return cm.d.dstack + (size_t)cm.n * r.idx;
+    default: return cm.d.xvec + (size_t)cm.n * r.idx;
+  }
+}
+
 // host <-> device matrix helpers for the unit surfaces of a single-chain context
 int put_mat(dqmc_batch* b, MatRef r, const double* host) {
   ChainMem& cm = b->chains[0];
   const size_t nn = (size_t)cm.n * cm.n;
-  cplx* dst = r.kind == MK_NAMED ? cm.d.mat[r.idx] : (r.kind == MK_USTACK ? cm.d.ustack : cm.d.tstack) + nn * r.idx;
-  HIPCHK(hipMemcpyAsync(dst, host, nn * sizeof(cplx), hipMemcpyHostToDevice, b->groups[0].stream));
+  HIPCHK(hipMemcpyAsync(host_matptr(cm, r), host, nn * sizeof(cplx), hipMemcpyHostToDevice, b->groups[0].stream));
   return 0;
 }
 int get_mat(dqmc_batch* b, MatRef r, double* host) {
   ChainMem& cm = b->chains[0];
   const size_t nn = (size_t)cm.n * cm.n;
-  cplx* src = r.kind == MK_NAMED ? cm.d.mat[r.idx] : (r.kind == MK_USTACK ? cm.d.ustack : cm.d.tstack) + nn * r.idx;
-  HIPCHK(hipMemcpyAsync(host, src, nn * sizeof(cplx), hipMemcpyDeviceToHost, b->groups[0].stream));
+  HIPCHK(hipMemcpyAsync(host, host_matptr(cm, r), nn * sizeof(cplx), hipMemcpyDeviceToHost, b->groups[0].stream));
   HIPCHK(hipStreamSynchronize(b->groups[0].stream));
   return 0;
 }
 int put_vec(dqmc_batch* b, VecRef r, const double* host) {
   ChainMem& cm = b->chains[0];
-  double* dst = r.kind == VK_NAMED ? cm.d.vec[r.idx] : cm.d.dstack + (size_t)cm.n * r.idx;
-  HIPCHK(hipMemcpyAsync(dst, host, cm.n * sizeof(double), hipMemcpyHostToDevice, b->groups[0].stream));
+  HIPCHK(hipMemcpyAsync(host_vecptr(cm, r), host, cm.n * sizeof(double), hipMemcpyHostToDevice,
+                        b->groups[0].stream));
   return 0;
 }
 int get_vec(dqmc_batch* b, VecRef r, double* host) {
   ChainMem& cm = b->chains[0];
-  double* src = r.kind == VK_NAMED ? cm.d.vec[r.idx] : cm.d.dstack + (size_t)cm.n * r.idx;
-  HIPCHK(hipMemcpyAsync(host, src, cm.n * sizeof(double), hipMemcpyDeviceToHost, b->groups[0].stream));
+  HIPCHK(hipMemcpyAsync(host, host_vecptr(cm, r), cm.n * sizeof(double), hipMemcpyDeviceToHost,
+                        b->groups[0].stream));
   HIPCHK(hipStreamSynchronize(b->groups[0].stream));
   return 0;
 }
@@ -1043,32 +1261,14 @@ int dqmc_greens_at(dqmc_ctx* ctx, int slice, double* G, double* logdet) {
   if (slice < 1 || slice > g.M) return fail(DQMC_E_ARG, "slice out of range");
   Ops o(g);
   const int sm = g.s;
-  // Ur, Dr, Tr = B(slice)' ... B(M)'  (calc_Bchain_dagger, decompose at k % sm == 0 and k == start)
-  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UR), NOVEC));
-  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TR), NOVEC));
-  LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DR)));
-  for (int k = g.M; k >= slice; --k) {
-    RC(o.applyB(NM(M_UR), B_DAG_LEFT, k - 1));
-    if (k % sm == 0 || k == slice) {
-      LCHK(launch_scale_cols(g.stream, g.d_desc, 1, g.n, NM(M_UR), NV(V_DR)));
-      RC(o.udt(NM(M_UR), NV(V_DR), NM(M_QRQ), NM(M_QRT)));
-      RC(o.copy(NM(M_UR), NM(M_QRQ)));
-      RC(o.gemm(NM(M_TMP), NM(M_QRT), OP_N, NM(M_TR), OP_N));
-      RC(o.copy(NM(M_TR), NM(M_TMP)));
-    }
-  }
-  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UL), NOVEC));
-  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TL), NOVEC));
-  LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DL)));
-  for (int k = 1; k <= slice - 1; ++k) {
-    RC(o.applyB(NM(M_UL), B_LEFT, k - 1));
-    if (k % sm == 0 || k == slice - 1) {
-      LCHK(launch_scale_cols(g.stream, g.d_desc, 1, g.n, NM(M_UL), NV(V_DL)));
-      RC(o.udt(NM(M_UL), NV(V_DL), NM(M_QRQ), NM(M_QRT)));
-      RC(o.copy(NM(M_UL), NM(M_QRQ)));
-      RC(o.gemm(NM(M_TMP), NM(M_QRT), OP_N, NM(M_TL), OP_N));
-      RC(o.copy(NM(M_TL), NM(M_TMP)));
-    }
+  // Ur, Dr, Tr = B(slice)' ... B(M)' (calc_Bchain_dagger); Ul, Dl, Tl = B(slice-1) ... B(1)
+  RC(o.bchain(slice, g.M, B_DAG_LEFT, NM(M_UR), NV(V_DR), NM(M_TR)));
+  if (slice > 1) {
+    RC(o.bchain(1, slice - 1, B_LEFT, NM(M_UL), NV(V_DL), NM(M_TL)));
+  } else {
+    LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UL), NOVEC));
+    LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TL), NOVEC));
+    LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DL)));
   }
   // keep the chain's G: compute into G then restore
   RC(o.copy(NM(M_GW), NM(M_G)));
@@ -1078,6 +1278,119 @@ int dqmc_greens_at(dqmc_ctx* ctx, int slice, double* G, double* logdet) {
   if (logdet) *logdet = read_scalar(ctx->b, 0, S_LOGDET);
   RC(o.copy(NM(M_G), NM(M_GW)));
   return batch_sync(ctx->b);
+}
+
+int dqmc_effective_greens2greens(dqmc_ctx* ctx, double* G) {
+  CTX_CHECK(ctx);
+  if (!G) return fail(DQMC_E_ARG, "null G");
+  Ops o(ctx->b->groups[0]);
+  RC(put_mat(ctx->b, NM(M_GRAD), G));
+  RC(o.effective_greens(NM(M_GRAD)));
+  RC(get_mat(ctx->b, NM(M_GRAD), G));
+  return check_device_error(ctx->b);
+}
+
+int dqmc_tdgf_alloc(dqmc_ctx* ctx) {
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  Group& g = b->groups[0];
+  ChainMem& cm = b->chains[0];
+  if (cm.tdgf_mat) return 0;
+  if (g.M % 2) return fail(DQMC_E_ARG, "time-displaced Green's functions need an even number of slices");
+  const size_t nn = (size_t)g.n * g.n;
+  const int R = g.M / g.s;
+  const size_t mat_bytes = ((size_t)8 * R + 2 * (size_t)g.M) * nn * sizeof(cplx);
+  const size_t vec_bytes = (size_t)4 * R * g.n * sizeof(double);
+  void *m = nullptr, *v = nullptr;
+  hipError_t e = hipMalloc(&m, mat_bytes);
+  if (e == hipSuccess) e = hipMalloc(&v, vec_bytes);
+  if (e == hipSuccess) e = hipMemset(m, 0, mat_bytes);
+  if (e == hipSuccess) e = hipMemset(v, 0, vec_bytes);
+  if (e != hipSuccess) {
+    if (m) (void)hipFree(m);
+    if (v) (void)hipFree(v);
+    return fail(DQMC_E_HIP, std::string("time-displaced Green's function storage: ") + hipGetErrorString(e));
+  }
+  cm.tdgf_mat = (cplx*)m;
+  cm.tdgf_vec = (double*)v;
+  cm.d.xmat = cm.tdgf_mat;
+  cm.d.xvec = cm.tdgf_vec;
+  return upload_desc(b, g);
+}
+
+int dqmc_tdgf_free(dqmc_ctx* ctx) {
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  ChainMem& cm = b->chains[0];
+  RC(batch_sync(b));
+  if (cm.tdgf_mat) (void)hipFree(cm.tdgf_mat);
+  if (cm.tdgf_vec) (void)hipFree(cm.tdgf_vec);
+  cm.tdgf_mat = nullptr;
+  cm.tdgf_vec = nullptr;
+  cm.d.xmat = nullptr;
+  cm.d.xvec = nullptr;
+  return upload_desc(b, b->groups[0]);
+}
+
+int dqmc_measure_tdgfs(dqmc_ctx* ctx) {
+  CTX_CHECK(ctx);
+  if (!ctx->b->chains[0].tdgf_mat) return fail(DQMC_E_STATE, "dqmc_tdgf_alloc has not been called");
+  Ops o(ctx->b->groups[0]);
+  RC(o.measure_tdgfs());
+  return batch_sync(ctx->b);
+}
+
+int dqmc_get_tdgf(dqmc_ctx* ctx, int tau, double* Gt0, double* G0t) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  if (!ctx->b->chains[0].tdgf_mat) return fail(DQMC_E_STATE, "dqmc_tdgf_alloc has not been called");
+  if (tau < 1 || tau > g.M) return fail(DQMC_E_ARG, "tau out of range");
+  Ops o(g);
+  if (Gt0) RC(get_mat(ctx->b, o.GT0(tau), Gt0));
+  if (G0t) RC(get_mat(ctx->b, o.G0T(tau), G0t));
+  return 0;
+}
+
+int dqmc_get_tdgf_stack(dqmc_ctx* ctx, int which, int k, double* U, double* D, double* T) {
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  if (!ctx->b->chains[0].tdgf_mat) return fail(DQMC_E_STATE, "dqmc_tdgf_alloc has not been called");
+  Ops o(g);
+  if (which < 0 || which > 3 || k < 0 || k >= o.nr()) return fail(DQMC_E_ARG, "bad stack index");
+  RC(batch_sync(ctx->b));
+  if (U) RC(get_mat(ctx->b, o.XU(which, k), U));
+  if (T) RC(get_mat(ctx->b, o.XT(which, k), T));
+  if (D) RC(get_vec(ctx->b, o.XD(which, k), D));
+  return 0;
+}
+
+int dqmc_tdgf_direct(dqmc_ctx* ctx, int slice, double* G) {
+  // calc_tdgf_direct (fermion_measurements.jl:1553-1576), scalettar form
+  CTX_CHECK(ctx);
+  Group& g = ctx->b->groups[0];
+  if (!G || slice < 1 || slice > g.M) return fail(DQMC_E_ARG, "bad tdgf_direct args");
+  Ops o(g);
+  if (slice != 1) {
+    RC(o.bchain(1, slice - 1, B_INV_LEFT, NM(M_UL), NV(V_DL), NM(M_TL)));
+  } else {
+    LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UL), NOVEC));
+    LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TL), NOVEC));
+    LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DL)));
+  }
+  if (slice != g.M) {
+    RC(o.bchain(slice, g.M, B_LEFT, NM(M_UR), NV(V_DR), NM(M_TR)));
+  } else {
+    LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UR), NOVEC));
+    LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TR), NOVEC));
+    LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DR)));
+  }
+  RC(o.inv_sum_udts_scalettar(MatRef{-1, 0}, NM(M_UL), NV(V_DL), NM(M_TL), NM(M_UR), NV(V_DR), NM(M_TR),
+                              NM(M_CURU), NV(V_D6), NM(M_GW)));
+  RC(o.effective_greens_ut(NM(M_CURU), NM(M_GW)));
+  LCHK(launch_scale_cols(g.stream, g.d_desc, 1, g.n, NM(M_CURU), NV(V_D6)));
+  RC(o.gemm(NM(M_GRAD), NM(M_CURU), OP_N, NM(M_GW), OP_N));
+  RC(batch_sync(ctx->b));
+  return get_mat(ctx->b, NM(M_GRAD), G);
 }
 
 int dqmc_detratio(dqmc_ctx* ctx, int site, const double* new_op, double* det, double* delta, double* Mout) {

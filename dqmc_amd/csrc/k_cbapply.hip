@@ -254,6 +254,19 @@ hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchai
       sl.st[3] = mkst(ST_PLAQ, MODE_T, PV_FULL_INV, 0, 0);
       sl.st[4] = mkst(ST_PLAQ, MODE_T, PV_HALF_INV, 1, 0);
       break;
+    case B_EFF_RIGHT:  // X hop_half[B] hop_half[A] (fermion_measurements.jl:1132-1135)
+      rows = true;
+      sl.count = 2;
+      sl.st[0] = mkst(ST_PLAQ, MODE_T, PV_HALF, 1, 0);
+      sl.st[1] = mkst(ST_PLAQ, MODE_T, PV_HALF, 0, 0);
+      break;
+    case B_EFF_INV_LEFT:  // hop_half_inv[A] hop_half_inv[B] X (fermion_measurements.jl:1136-1139)
+      sl.count = 2;
+      sl.st[0] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 1, 0);
+      sl.st[1] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 0, 0);
+      break;
+    default:
+      return hipErrorInvalidValue;
   }
   return run_stages(s, chains, nchains, nmax, Nmax, X, rows, sl, slice, slices);
 }

@@ -243,7 +243,16 @@ hipError_t launch_gemm(hipStream_t s, const ChainDesc* chains, int nchains, int 
 // ---------------------------------------------------------------------------
 // Elementwise kernels: grid (blocks, nchains), grid-stride over n*n (or n).
 // ---------------------------------------------------------------------------
-enum EwOp : int { EW_COPY = 0, EW_IDENT, EW_SCALE_COLS, EW_ADD_DIAG, EW_ADJOINT, EW_SCALE_ROWS_COPY };
+enum EwOp : int {
+  EW_COPY = 0,
+  EW_IDENT,
+  EW_SCALE_COLS,
+  EW_ADD_DIAG,
+  EW_ADJOINT,
+  EW_SCALE_ROWS_COPY,
+  EW_ADD_SCALED_COLS,
+  EW_NEGATE
+};
 
 __global__ __launch_bounds__(256) void k_elementwise(const ChainDesc* chains, int op, MatRef Dr, MatRef Sr, VecRef vr) {
   const ChainDesc& c = chains[blockIdx.y];
@@ -276,6 +285,15 @@ __global__ __launch_bounds__(256) void k_elementwise(const ChainDesc* chains, in
       const size_t col = e / n, row = e - col * n;
       D[e] = d[row] * S[e];
     }
+  } else if (op == EW_ADD_SCALED_COLS) {
+    const cplx* S = matptr(c, Sr);
+    const double* d = vecptr(c, vr);
+    for (size_t e = start; e < nn; e += stride) {
+      const size_t col = e / n;
+      D[e] = D[e] + d[col] * S[e];
+    }
+  } else if (op == EW_NEGATE) {
+    for (size_t e = start; e < nn; e += stride) D[e] = -D[e];
   } else if (op == EW_ADJOINT) {
     const cplx* S = matptr(c, Sr);
     for (size_t e = start; e < nn; e += stride) {
@@ -295,6 +313,13 @@ static hipError_t ew(hipStream_t s, const ChainDesc* chains, int nchains, int nm
   return hipGetLastError();
 }
 
+hipError_t launch_add_scaled_cols(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X,
+                                  MatRef src, VecRef d) {
+  return ew(s, chains, nchains, nmax, EW_ADD_SCALED_COLS, X, src, d);
+}
+hipError_t launch_negate(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X) {
+  return ew(s, chains, nchains, nmax, EW_NEGATE, X, X, VecRef{0, 0});
+}
 hipError_t launch_copy(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef dst, MatRef src) {
   return ew(s, chains, nchains, nmax, EW_COPY, dst, src, VecRef{0, 0});
 }

@@ -338,6 +338,52 @@ class DQMC:
         check(self.lib.dqmc_boson_action(self.h, ctypes.byref(s)))
         return s.value
 
+    # -- measurement hand-off (fermion_measurements.jl:1125-1193, 1239-1576) ---------------
+    def effective_greens2greens(self, G):
+        """effective_greens2greens (fermion_measurements.jl:1204-1208): a new matrix."""
+        X = _as_fcomplex(G, self.p.n).copy(order="F")
+        check(self.lib.dqmc_effective_greens2greens(self.h, _cmat_ptr(X)))
+        return X
+
+    def allocate_tdgfs(self):
+        """allocate_tdgfs! (fermion_measurements.jl:1239-1265): device storage."""
+        check(self.lib.dqmc_tdgf_alloc(self.h))
+
+    def deallocate_tdgfs(self):
+        check(self.lib.dqmc_tdgf_free(self.h))
+
+    def measure_tdgfs(self):
+        """measure_tdgfs! (fermion_measurements.jl:1343-1407); results stay on the device,
+        read them with tdgf(tau)."""
+        check(self.lib.dqmc_measure_tdgfs(self.h))
+
+    def tdgf(self, tau):
+        """(Gt0[tau], G0t[tau]) for 1 <= tau <= M."""
+        n = self.p.n
+        a = np.zeros((n, n), dtype=np.complex128, order="F")
+        b = np.zeros((n, n), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_get_tdgf(self.h, tau, _cmat_ptr(a), _cmat_ptr(b)))
+        return a, b
+
+    TDGF_STACKS = ("BT0Inv", "BBetaT", "BT0", "BBetaTInv")
+
+    def tdgf_stack(self, name, k):
+        """Entry k (0-based) of one of the four checkpoint chains as (U, D, T)."""
+        n = self.p.n
+        U = np.zeros((n, n), dtype=np.complex128, order="F")
+        T = np.zeros((n, n), dtype=np.complex128, order="F")
+        D = np.zeros(n)
+        check(self.lib.dqmc_get_tdgf_stack(self.h, self.TDGF_STACKS.index(name), k, _cmat_ptr(U), dptr(D),
+                                           _cmat_ptr(T)))
+        return U, D, T
+
+    def calc_tdgf_direct(self, slice_):
+        """calc_tdgf_direct (fermion_measurements.jl:1553-1576)."""
+        n = self.p.n
+        G = np.zeros((n, n), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_tdgf_direct(self.h, slice_, _cmat_ptr(G)))
+        return G
+
     def grad(self):
         """dS/dphi (3, N, M) and S = S_B + log|det G| of the current field."""
         g = np.zeros((3, self.p.N, self.p.slices), order="F")

@@ -166,6 +166,28 @@ int dqmc_inverse(dqmc_ctx* ctx, const double* A, double* Ainv, double* logabsdet
 /* calc_boson_action (action.jl:1-52) of the current field */
 int dqmc_boson_action(dqmc_ctx* ctx, double* S);
 
+/* ---- measurement hand-off -------------------------------------------------- */
+/* effective_greens2greens! (fermion_measurements.jl:1125-1141 checkerboard,
+ * :1178-1185 dense) on an n x n host matrix, in place */
+int dqmc_effective_greens2greens(dqmc_ctx* ctx, double* G);
+/* allocate_tdgfs! (fermion_measurements.jl:1239-1265): device storage for Gt0/G0t
+ * at every slice (2 M n^2 complex) and the four checkpoint chains (8 M/safe_mult
+ * n x n complex). Needs an even M (fill_tdgf! splits at M/2 + 1). Values persist
+ * between measurements, zero after allocation. */
+int dqmc_tdgf_alloc(dqmc_ctx* ctx);
+/* deallocate_tdgfs_stacks! (fermion_measurements.jl:1267-1284); frees Gt0/G0t too */
+int dqmc_tdgf_free(dqmc_ctx* ctx);
+/* measure_tdgfs! (fermion_measurements.jl:1343-1407, fill_tdgf! :1513-1541) for
+ * the current field; results stay on the device */
+int dqmc_measure_tdgfs(dqmc_ctx* ctx);
+/* mc.s.meas.Gt0[tau], mc.s.meas.G0t[tau] (tau 1-based); either pointer may be NULL */
+int dqmc_get_tdgf(dqmc_ctx* ctx, int tau, double* Gt0, double* G0t);
+/* entry k (0-based) of BT0Inv (which 0), BBetaT (1), BT0 (2), BBetaTInv (3)
+ * _u/_d/_t_stack (fermion_measurements.jl:1248-1259); any pointer may be NULL */
+int dqmc_get_tdgf_stack(dqmc_ctx* ctx, int which, int k, double* U, double* D, double* T);
+/* calc_tdgf_direct (fermion_measurements.jl:1553-1576): G(slice, 1) from scratch */
+int dqmc_tdgf_direct(dqmc_ctx* ctx, int slice, double* G);
+
 /* No reference counterpart: dS/dphi for every (component, site, slice) of
  * S(phi) = S_B(phi) - log det(1 + B_M...B_1), computed from the Green's
  * functions of one stabilised pass over all slices. Also returns S (optional). */
