@@ -33,7 +33,8 @@ class Stats(ctypes.Structure):
                 ("drift_last", ctypes.c_double), ("drift_max", ctypes.c_double),
                 ("acc_last", ctypes.c_double), ("accepted", ctypes.c_double), ("proposed", ctypes.c_double),
                 ("nonreal", ctypes.c_double), ("min_ratio_gap", ctypes.c_double),
-                ("uniforms_used", ctypes.c_longlong), ("device_error", ctypes.c_int)]
+                ("uniforms_used", ctypes.c_longlong), ("device_error", ctypes.c_int),
+                ("global_accepted", ctypes.c_double), ("global_proposed", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p
@@ -76,6 +77,7 @@ SIGNATURES = {
     "dqmc_gemm": [_P, _i, _i, _D, _D, _D],
     "dqmc_inverse": [_P, _D, _D, _D],
     "dqmc_boson_action": [_P, _D],
+    "dqmc_global_update": [_P, _d, _I],
     "dqmc_effective_greens2greens": [_P, _D],
     "dqmc_tdgf_alloc": [_P],
     "dqmc_tdgf_free": [_P],

@@ -107,7 +107,14 @@ enum ScalarId : int {
   S_NONREAL,           // proposals with |Im r / Re r| > 1e-4
   S_MINRATIO,          // min |p_acc - u| over proposals that drew u
   S_DRIFT_TMP,         // accumulator of the max-abs difference kernel
-  kNumScalar = 16
+  S_GB_ACTION,         // boson action before the pending global update
+  S_GB_LOGDET,         // log|det G| before the pending global update (mc.s.gb_log_det)
+  S_GLOBAL_ACC,        // 1 if the last global update was accepted
+  S_GLOBAL_PACC,       // its acceptance probability
+  S_GLOBAL_NACC,       // accepted global updates since reset
+  S_GLOBAL_NPROP,      // global updates since reset
+  S_LOGDET_SAVE,       // S_LOGDET kept across the unit surfaces that recompute G
+  kNumScalar = 24
 };
 
 __device__ inline cplx* matptr(const ChainDesc& c, MatRef r) {
@@ -212,6 +219,17 @@ hipError_t launch_boson_action(hipStream_t s, const ChainDesc* chains, int nchai
 hipError_t launch_boson_grad(hipStream_t s, const ChainDesc* chains, int nchains, int NMmax, double* const* grad);
 hipError_t launch_daxpy(hipStream_t s, double* y, const double* x, double a, long long n);
 // dS/dphi fermion part at one slice from G (slice 0-based): grad[3][N] += -Re tr[(1-G)_ii eV^-1 d eV]
+// global update (global_updates.jl:1-58) of chains[0]
+struct RestoreList {
+  double* dst[6];
+  const double* src[6];
+  long long count[6];  // doubles
+  int n;
+};
+hipError_t launch_global_shift(hipStream_t s, const ChainDesc* chain, double* phi_bak, double box);
+hipError_t launch_global_decide(hipStream_t s, const ChainDesc* chain);
+hipError_t launch_global_restore(hipStream_t s, const double* flag, const RestoreList& r);
+
 hipError_t launch_grad_slice(hipStream_t s, const ChainDesc* chains, int nchains, int Nmax, int slice,
                              double* const* grad);
 

@@ -63,6 +63,8 @@ struct ChainMem {
   // time-displaced Green's function storage (dqmc_tdgf_alloc)
   cplx* tdgf_mat = nullptr;
   double* tdgf_vec = nullptr;
+  // global-update backup (gb_u/d/t_stack, gb_greens, gb_hsfield; stack.jl:134-143), one block
+  double* gb = nullptr;
 };
 
 struct Group {
@@ -277,10 +279,11 @@ void chain_free(ChainMem& cm) {
   if (cm.uniforms) (void)hipFree(cm.uniforms);
   if (cm.log_buf) (void)hipFree(cm.log_buf);
   for (void* p : {(void*)cm.grad, (void*)cm.grad_ptr, (void*)cm.mom, (void*)cm.phi_bak, (void*)cm.tdgf_mat,
-                  (void*)cm.tdgf_vec})
+                  (void*)cm.tdgf_vec, (void*)cm.gb})
     if (p) (void)hipFree(p);
   cm.tdgf_mat = nullptr;
   cm.tdgf_vec = nullptr;
+  cm.gb = nullptr;
   cm.uniforms = nullptr;
   cm.log_buf = nullptr;
   cm.grad = nullptr;
@@ -947,6 +950,8 @@ int get_stats(dqmc_batch* b, int chain, dqmc_stats* st) {
   st->min_ratio_gap = sc[S_MINRATIO];
   st->uniforms_used = cur;
   st->device_error = err;
+  st->global_accepted = sc[S_GLOBAL_NACC];
+  st->global_proposed = sc[S_GLOBAL_NPROP];
   (void)rc;
   return 0;
 }
@@ -1270,14 +1275,77 @@ int dqmc_greens_at(dqmc_ctx* ctx, int slice, double* G, double* logdet) {
     LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TL), NOVEC));
     LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DL)));
   }
-  // keep the chain's G: compute into G then restore
+  // keep the chain's G and log_det: compute into them, then restore
+  double* sc = ctx->b->chains[0].d.scal;
+  HIPCHK(hipMemcpyAsync(sc + S_LOGDET_SAVE, sc + S_LOGDET, sizeof(double), hipMemcpyDeviceToDevice, g.stream));
   RC(o.copy(NM(M_GW), NM(M_G)));
   RC(o.calculate_greens(true));
   RC(batch_sync(ctx->b));
   if (G) RC(get_mat(ctx->b, NM(M_G), G));
   if (logdet) *logdet = read_scalar(ctx->b, 0, S_LOGDET);
   RC(o.copy(NM(M_G), NM(M_GW)));
+  HIPCHK(hipMemcpyAsync(sc + S_LOGDET, sc + S_LOGDET_SAVE, sizeof(double), hipMemcpyDeviceToDevice, g.stream));
   return batch_sync(ctx->b);
+}
+
+int dqmc_global_update(dqmc_ctx* ctx, double box_global, int* accepted) {
+  // global_update (global_updates.jl:19-58). The reference swaps host stacks into the
+  // backup (:1-8); here the backup is a device copy and the restore on rejection is a
+  // kernel conditioned on the device-side decision, so nothing waits for the host.
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  Group& g = b->groups[0];
+  ChainMem& cm = b->chains[0];
+  if (g.cur != g.M || g.dir != -1)
+    return fail(DQMC_E_STATE, "global_update needs current_slice == M and direction == -1");
+  if (!(box_global >= 0.0)) return fail(DQMC_E_ARG, "box_global must be >= 0");
+  const size_t nn = (size_t)g.n * g.n;
+  const size_t n_st = 2 * nn * cm.nelem;  // doubles of one of u_stack / t_stack
+  const size_t n_d = (size_t)g.n * cm.nelem, n_g = 2 * nn, n_phi = (size_t)3 * g.N * g.M;
+  if (!cm.gb) HIPCHK(hipMalloc(&cm.gb, sizeof(double) * (2 * n_st + n_d + n_g + n_phi)));
+  double* gu = cm.gb;
+  double* gt = gu + n_st;
+  double* gd = gt + n_st;
+  double* gg = gd + n_d;
+  double* gphi = gg + n_g;
+  hipStream_t st = g.stream;
+  ChainDesc& d = cm.d;
+  HIPCHK(hipMemcpyAsync(gu, d.ustack, sizeof(double) * n_st, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(gt, d.tstack, sizeof(double) * n_st, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(gd, d.dstack, sizeof(double) * n_d, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(gg, d.mat[M_G], sizeof(double) * n_g, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(d.scal + S_GB_ACTION, d.scal + S_BOSON_ACTION, sizeof(double), hipMemcpyDeviceToDevice,
+                        st));
+  HIPCHK(hipMemcpyAsync(d.scal + S_GB_LOGDET, d.scal + S_LOGDET, sizeof(double), hipMemcpyDeviceToDevice, st));
+  LCHK(launch_global_shift(st, g.d_desc, gphi, box_global));
+  Ops o(g);
+  RC(o.build_stack());
+  RC(o.propagate());
+  LCHK(launch_boson_action(st, g.d_desc, 1, g.N));
+  LCHK(launch_global_decide(st, g.d_desc));
+  RestoreList r;
+  r.n = 5;
+  r.dst[0] = (double*)d.ustack;
+  r.src[0] = gu;
+  r.count[0] = (long long)n_st;
+  r.dst[1] = (double*)d.tstack;
+  r.src[1] = gt;
+  r.count[1] = (long long)n_st;
+  r.dst[2] = d.dstack;
+  r.src[2] = gd;
+  r.count[2] = (long long)n_d;
+  r.dst[3] = (double*)d.mat[M_G];
+  r.src[3] = gg;
+  r.count[3] = (long long)n_g;
+  r.dst[4] = d.phi;
+  r.src[4] = gphi;
+  r.count[4] = (long long)n_phi;
+  LCHK(launch_global_restore(st, d.scal + S_GLOBAL_ACC, r));
+  if (accepted) {
+    RC(batch_sync(b));
+    *accepted = (int)read_scalar(b, 0, S_GLOBAL_ACC);
+  }
+  return 0;
 }
 
 int dqmc_effective_greens2greens(dqmc_ctx* ctx, double* G) {

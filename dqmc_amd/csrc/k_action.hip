@@ -166,4 +166,94 @@ hipError_t launch_grad_slice(hipStream_t s, const ChainDesc* chains, int nchains
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Global update (global_updates.jl:1-58): uniform shift of every component,
+// decision, conditional restore. One chain per launch (chains[0]).
+// ---------------------------------------------------------------------------
+
+// phi_bak <- phi; phi[a, :, :] += -b + 2 b u_a with the next three uniforms
+// (global_update_perform_shift!, global_updates.jl:10-16; randuniform, dqmc_framework.jl:628)
+__global__ __launch_bounds__(256) void k_global_shift(const ChainDesc* chains, double* phi_bak, double box) {
+  const ChainDesc& c = chains[0];
+  __shared__ double sh[3];
+  __shared__ int ok;
+  if (threadIdx.x == 0) {
+    const long long cur = *c.ucursor;
+    ok = (cur + 3 <= c.n_uniforms) ? 1 : 0;
+    if (ok) {
+      for (int a = 0; a < 3; ++a) sh[a] = -box + 2 * box * c.uniforms[cur + a];
+    } else {
+      *c.error = 2;
+    }
+  }
+  __syncthreads();
+  if (!ok) return;
+  const long long tot = 3ll * c.N * c.M;
+  for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+    const double v = c.phi[e];
+    phi_bak[e] = v;
+    c.phi[e] = v + sh[e % 3];
+  }
+  if (threadIdx.x == 0) *c.ucursor += 3;
+}
+
+// p_acc = exp(-(S_B new - S_B old)) exp(log_det old - log_det new); accept iff p_acc > 1 or
+// u < p_acc with one more uniform (global_updates.jl:38-57). On rejection the scalars are restored.
+__global__ void k_global_decide(const ChainDesc* chains) {
+  const ChainDesc& c = chains[0];
+  if (threadIdx.x != 0) return;
+  double* sc = c.scal;
+  const double S_old = sc[S_GB_ACTION], ld_old = sc[S_GB_LOGDET];
+  const double p_boson = exp(-(sc[S_BOSON_ACTION] - S_old));
+  const double p_fermion = exp(ld_old - sc[S_LOGDET]);
+  const double p_acc = p_boson * p_fermion;
+  int acc = 1;
+  double u = -1.0;
+  if (!(p_acc > 1.0)) {
+    const long long cur = *c.ucursor;
+    if (cur < c.n_uniforms) {
+      u = c.uniforms[cur];
+      *c.ucursor = cur + 1;
+      acc = u < p_acc ? 1 : 0;
+    } else {
+      *c.error = 2;
+      acc = 0;
+    }
+  }
+  sc[S_GLOBAL_ACC] = acc;
+  sc[S_GLOBAL_PACC] = p_acc;
+  sc[S_GLOBAL_NACC] += acc;
+  sc[S_GLOBAL_NPROP] += 1.0;
+  if (u >= 0.0) sc[S_MINRATIO] = fmin(sc[S_MINRATIO], fabs(p_acc - u));
+  if (!acc) {
+    sc[S_BOSON_ACTION] = S_old;
+    sc[S_LOGDET] = ld_old;
+  }
+}
+
+// dst_k <- src_k for every listed buffer iff the last global update was rejected
+__global__ __launch_bounds__(256) void k_global_restore(const double* flag, RestoreList r) {
+  if (*flag != 0.0) return;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (int k = 0; k < r.n; ++k) {
+    double* dst = r.dst[k];
+    const double* src = r.src[k];
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < r.count[k]; e += stride)
+      dst[e] = src[e];
+  }
+}
+
+hipError_t launch_global_shift(hipStream_t s, const ChainDesc* chain, double* phi_bak, double box) {
+  hipLaunchKernelGGL(k_global_shift, dim3(1), dim3(256), 0, s, chain, phi_bak, box);
+  return hipGetLastError();
+}
+hipError_t launch_global_decide(hipStream_t s, const ChainDesc* chain) {
+  hipLaunchKernelGGL(k_global_decide, dim3(1), dim3(64), 0, s, chain);
+  return hipGetLastError();
+}
+hipError_t launch_global_restore(hipStream_t s, const double* flag, const RestoreList& r) {
+  hipLaunchKernelGGL(k_global_restore, dim3(1024), dim3(256), 0, s, flag, r);
+  return hipGetLastError();
+}
+
 }  // namespace dqmc

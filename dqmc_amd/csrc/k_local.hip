@@ -447,6 +447,8 @@ __global__ void k_reset_stats(const ChainDesc* chains) {
     c.scal[S_DRIFT_MAX] = 0;
     c.scal[S_NONREAL] = 0;
     c.scal[S_MINRATIO] = 1e300;
+    c.scal[S_GLOBAL_NACC] = 0;
+    c.scal[S_GLOBAL_NPROP] = 0;
     *c.log_count = 0;
   }
 }

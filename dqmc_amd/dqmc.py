@@ -165,9 +165,23 @@ class DQMC:
         self._consume_uniforms()
         return acc.value
 
-    def update(self, box=None):
-        """update (dqmc_framework.jl:500-517) without global updates."""
-        self.propagate()
+    def global_update(self, box_global=None):
+        """global_update (global_updates.jl:19-58) at current_slice == M, direction == -1;
+        returns 1 if accepted, else 0 (field, stack, G, log_det and boson action restored)."""
+        b = self.p.box_global if box_global is None else box_global
+        self._upload_uniforms(4)
+        acc = ctypes.c_int()
+        check(self.lib.dqmc_global_update(self.h, b, ctypes.byref(acc)))
+        self._consume_uniforms()
+        return acc.value
+
+    def update(self, i=1, box=None):
+        """update(mc, i) (dqmc_framework.jl:500-517): propagate, a global update when
+        p.global_updates and current_slice == M, direction == -1 and i % global_rate == 0,
+        then local updates. i is the ud-sweep counter of thermalize!/measure!."""
+        cs, d, _ = self.propagate()
+        if self.p.global_updates and cs == self.p.slices and d == -1 and i % self.p.global_rate == 0:
+            self.global_update()
         return self.local_updates(box)
 
     def sweep(self, nsweeps=1, box=None, sync=True):

@@ -84,6 +84,8 @@ typedef struct {
   double min_ratio_gap;      /* min |p_acc - u| over proposals that drew u */
   long long uniforms_used;   /* position of the uniform stream cursor */
   int device_error;          /* nonzero after a device-side failure */
+  double global_accepted;    /* accepted global updates since reset (a.acc_global) */
+  double global_proposed;    /* global updates since reset (a.prop_global) */
 } dqmc_stats;
 
 const char* dqmc_last_error(void);
@@ -165,6 +167,14 @@ int dqmc_gemm(dqmc_ctx* ctx, int opA, int opB, const double* A, const double* B,
 int dqmc_inverse(dqmc_ctx* ctx, const double* A, double* Ainv, double* logabsdet);
 /* calc_boson_action (action.jl:1-52) of the current field */
 int dqmc_boson_action(dqmc_ctx* ctx, double* S);
+
+/* global_update (global_updates.jl:19-58), callable when current_slice == M and
+ * direction == -1 (update, dqmc_framework.jl:507-512): shifts every component of
+ * phi by -b + 2 b u with three uniforms from the stream (b = box_global), rebuilds
+ * the stack, and accepts with p = exp(-dS_B) exp(log_det_old - log_det_new) (one
+ * more uniform when p <= 1). On rejection field, stack, G, log_det and boson
+ * action are restored. `accepted` is optional; passing it synchronises. */
+int dqmc_global_update(dqmc_ctx* ctx, double box_global, int* accepted);
 
 /* ---- measurement hand-off -------------------------------------------------- */
 /* effective_greens2greens! (fermion_measurements.jl:1125-1141 checkerboard,

@@ -840,9 +840,12 @@ class DQMC:
                 self.update_greens(i)
         return acc / p.N
 
-    def update(self, stream, log=None):
-        """dqmc_framework.jl:500-517 without global updates."""
+    def update(self, stream, log=None, i=1, global_rate=0, box_global=None):
+        """update(mc, i) (dqmc_framework.jl:500-517); global updates when global_rate > 0."""
         self.propagate()
+        p = self.p
+        if global_rate and self.current_slice == p.slices and self.direction == -1 and i % global_rate == 0:
+            self.global_update(stream, p.box if box_global is None else box_global)
         return self.local_updates(stream, log)
 
     def sweep(self, stream, log=None):
@@ -869,7 +872,11 @@ class DQMC:
         p_boson = math.exp(-(self.boson_action - S_old))
         p_fermion = math.exp(backup[4] - self.log_det)
         p_acc = p_boson * p_fermion
-        if p_acc > 1.0 or stream.next() < p_acc:
+        u = None
+        if not p_acc > 1.0:
+            u = stream.next()
+        self.last_global = (p_acc, u)
+        if p_acc > 1.0 or u < p_acc:
             return 1
         self.boson_action = S_old
         self.u_stack, self.d_stack, self.t_stack, self.greens, self.log_det, self.m.phi = backup
