@@ -19,12 +19,20 @@
 
 namespace dqmc {
 
-__device__ inline void ev_op(double lam_dtau, double o0, double o1, double o2, double power, cplx E[16]) {
+// coefficients of interaction_matrix_exp_op (interactions.jl:102-140)
+__device__ inline void ev_coeffs_op(double lam_dtau, double o0, double o1, double o2, double power, double& C,
+                                    cplx& S, double& R) {
   const double nrm = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
   const double sh = power * sinh(lam_dtau * nrm) / nrm;
-  const double C = cosh(lam_dtau * nrm);
-  const cplx S = mk(-o0 * sh, o1 * sh);
-  const double R = -o2 * sh;
+  C = cosh(lam_dtau * nrm);
+  S = mk(-o0 * sh, o1 * sh);
+  R = -o2 * sh;
+}
+
+__device__ inline void ev_op(double lam_dtau, double o0, double o1, double o2, double power, cplx E[16]) {
+  double C, R;
+  cplx S;
+  ev_coeffs_op(lam_dtau, o0, o1, o2, power, C, S, R);
   const cplx z = mk(0, 0);
   E[0] = mk(C, 0);   E[1] = S;          E[2] = z;          E[3] = mk(R, 0);
   E[4] = conj(S);    E[5] = mk(C, 0);   E[6] = mk(-R, 0);  E[7] = z;
@@ -129,10 +137,29 @@ __device__ inline void inv4(const cplx* Min, cplx* out) {
 struct LocalShared {
   cplx delta[16];
   cplx minv[16];
-  int accepted;
-  int stop;
+  int site;  // site of the accepted proposal to apply, N when the slice is done
+  int stop;  // the uniform stream ran out
 };
 
+// entry e (row-major) of the site block [[C,S,0,R],[S*,C,-R,0],[0,-R,C,S*],[R,0,S,C]]
+__device__ inline cplx ev_entry(double C, cplx S, double R, int e) {
+  switch (e) {
+    case 0: case 5: case 10: case 15: return mk(C, 0);
+    case 1: case 14: return S;
+    case 4: case 11: return conj(S);
+    case 3: case 12: return mk(R, 0);
+    case 6: case 9: return mk(-R, 0);
+    default: return mk(0, 0);
+  }
+}
+
+__device__ inline cplx shfl_c(cplx v, int src) {
+  return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64));
+}
+
+// Proposals are evaluated by wave 0 alone, every lane holding one entry (a, b) of the 4x4
+// algebra (lanes 16..63 mirror 0..15); the other waves wait at one workgroup barrier per
+// accepted proposal, so a rejected proposal synchronises nothing.
 __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, int slice, const int* slices,
                                                        double box, int dolog) {
   const ChainDesc& c = chains[blockIdx.y];
@@ -150,11 +177,12 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
   double* pla = phe + 3 * N;       // [3N] later slice
   int* nbl = reinterpret_cast<int*>(pla + 3 * N);  // [4N]
   double* ul = smem + 9 * N + 2 * N;               // [4N] this slice's uniforms
-  cplx* Acol = reinterpret_cast<cplx*>(smem + 13 * N + 2 * N + (N & 1));  // [n][4]
+  double* e1c = ul + 4 * N;                        // [4N] C, S.re, S.im, R of eV_op(phi_i, -1)
+  cplx* Acol = reinterpret_cast<cplx*>(smem + 17 * N + 2 * N + (N & 1));  // [n][4]
   cplx* Brow = Acol + 4 * (size_t)n;                                       // [4][ncol]
   __shared__ LocalShared sh;
-  __shared__ cplx gblk[16];
 
+  const double lam_dtau = c.lam * c.dtau;
   const int le = (l == 0) ? M - 1 : l - 1;  // time_neighbors (lattice.jl:144-153)
   const int la = (l == M - 1) ? 0 : l + 1;
   for (int e = tid; e < 3 * N; e += blockDim.x) {
@@ -167,27 +195,42 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
   const long long cur0 = *c.ucursor;
   const long long navail = min((long long)4 * N, c.n_uniforms - cur0);
   for (long long e = tid; e < navail; e += blockDim.x) ul[e] = c.uniforms[cur0 + e];
+  // each site is visited once per slice, so its old field is the value at kernel start
+  for (int i = tid; i < N; i += blockDim.x) {
+    const double* f = c.phi + (size_t)3 * N * l + 3 * i;
+    double C, R;
+    cplx S;
+    ev_coeffs_op(lam_dtau, f[0], f[1], f[2], -1.0, C, S, R);
+    e1c[4 * i + 0] = C;
+    e1c[4 * i + 1] = S.re;
+    e1c[4 * i + 2] = S.im;
+    e1c[4 * i + 3] = R;
+  }
   __syncthreads();
 
-  const double lam_dtau = c.lam * c.dtau;
+  const int lane = tid & 63;
+  const int ea = (lane & 15) >> 2, eb = lane & 3;  // this lane's entry of the 4x4 algebra
   long long cur = cur0;
   long long logi = dolog ? *c.log_count : 0;
   double dS_acc = 0.0;
   int nacc = 0, nonreal = 0;
   double minratio = c.scal[S_MINRATIO];
-  int buf = 0;
+  int buf = 0, i = 0;
   cplx* Gb[2] = {c.mat[M_G], c.mat[M_GALT]};
 
-  for (int i = 0; i < N; ++i) {
-    // the 4x4 site block of the current G, one element per lane
-    if (tid < 16) gblk[tid] = Gb[buf][(size_t)(i + (tid & 3) * N) * n + (i + (tid >> 2) * N)];
-    __syncthreads();
-    if (tid == 0) {
-      sh.accepted = 0;
-      sh.stop = 0;
-      if ((cur - cur0) + 4 > navail) {
-        sh.stop = 1;
-      } else {
+  for (;;) {
+    if (tid < 64) {
+      int site = N, stop = 0;
+      // G_ii(a, b) of the site under evaluation, loaded one site ahead (valid while G is unchanged)
+      cplx gnext = mk(0, 0);
+      if (i < N) gnext = Gb[buf][(size_t)(i + eb * N) * n + (i + ea * N)];
+      for (; i < N; ++i) {
+        const cplx gv = gnext;
+        if (i + 1 < N) gnext = Gb[buf][(size_t)(i + 1 + eb * N) * n + (i + 1 + ea * N)];
+        if ((cur - cur0) + 4 > navail) {
+          stop = 1;
+          break;
+        }
         const double* u = ul + (cur - cur0);
         double old[3], nw[3];
         for (int a = 0; a < 3; ++a) {
@@ -217,21 +260,26 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
           dS = c.dtau * (0.5 * c.r * sq_diff);
         }
         const double exp_dS = exp(-dS);
-        // calc_detratio (local_updates.jl:42-59)
-        cplx E1[16], E2[16], Dm[16], Mm[16];
-        ev_op(lam_dtau, old[0], old[1], old[2], -1.0, E1);
-        ev_op(lam_dtau, nw[0], nw[1], nw[2], 1.0, E2);
-        mul4(E1, E2, Dm);
-        for (int a = 0; a < 4; ++a) Dm[a * 4 + a].re -= 1.0;
-        cplx Gm[16];
-        for (int a = 0; a < 4; ++a)
-          for (int b = 0; b < 4; ++b) {
-            const cplx gv = gblk[a * 4 + b];
-            Gm[a * 4 + b] = mk((a == b ? 1.0 : 0.0) - gv.re, -gv.im);  // I - G_ii
-          }
-        mul4(Dm, Gm, Mm);
-        for (int a = 0; a < 4; ++a) Mm[a * 4 + a].re += 1.0;
-        const cplx det = det4(Mm);
+        // calc_detratio (local_updates.jl:42-59): delta = eV_op(old, -1) eV_op(new, +1) - I,
+        // M = I + delta (I - G_ii)
+        double C2, R2;
+        cplx S2;
+        ev_coeffs_op(lam_dtau, nw[0], nw[1], nw[2], 1.0, C2, S2, R2);
+        const double C1 = e1c[4 * i], R1 = e1c[4 * i + 3];
+        const cplx S1 = mk(e1c[4 * i + 1], e1c[4 * i + 2]);
+        cplx dm = mk(0, 0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cfma(dm, ev_entry(C1, S1, R1, ea * 4 + k), ev_entry(C2, S2, R2, k * 4 + eb));
+        if (ea == eb) dm.re -= 1.0;
+        const cplx gm = mk((ea == eb ? 1.0 : 0.0) - gv.re, -gv.im);
+        cplx mm = mk(0, 0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cfma(mm, shfl_c(dm, ea * 4 + k), shfl_c(gm, k * 4 + eb));
+        if (ea == eb) mm.re += 1.0;
+        cplx Mall[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) Mall[q] = shfl_c(mm, q);
+        const cplx det = det4(Mall);
         if (fabs(det.im / det.re) > 1e-4) ++nonreal;
         const double p_acc = exp_dS * det.re;
         int acc;
@@ -245,7 +293,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
           cur += 4;
           minratio = fmin(minratio, fabs(p_acc - uval));
         }
-        if (dolog && w == 0 && logi < c.log_cap) {
+        if (dolog && w == 0 && lane == 0 && logi < c.log_cap) {
           double* lg = c.prop_log + 4 * logi;
           lg[0] = p_acc;
           lg[1] = uval;
@@ -256,24 +304,34 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
         if (acc) {
           ++nacc;
           dS_acc += -log(exp_dS);
-          for (int a = 0; a < 3; ++a) phl[3 * i + a] = nw[a];
-          for (int e = 0; e < 16; ++e) sh.delta[e] = Dm[e];
-          inv4(Mm, sh.minv);
-          sh.accepted = 1;
+          if (lane == 0)
+            for (int a = 0; a < 3; ++a) phl[3 * i + a] = nw[a];
+          if (lane < 16) sh.delta[lane] = dm;
+          cplx Mi[16];
+          inv4(Mall, Mi);
+          if (lane == 0)
+            for (int q = 0; q < 16; ++q) sh.minv[q] = Mi[q];
+          site = i;
+          ++i;
+          break;
         }
+      }
+      if (lane == 0) {
+        sh.site = site;
+        sh.stop = stop;
       }
     }
     __syncthreads();
-    if (sh.stop) break;
-    if (!sh.accepted) continue;
+    const int site = sh.site;
+    if (site >= N) break;
     const cplx* Gc = Gb[buf];
     cplx* Gn = Gb[buf ^ 1];
-    // A = (G[:, c_i] - E) M^-1   (n x 4)
+    // update_greens! (local_updates.jl:61-95): A = (G[:, c_i] - E) M^-1 (n x 4)
     for (int r = tid; r < n; r += blockDim.x) {
       cplx g4[4];
       for (int k = 0; k < 4; ++k) {
-        g4[k] = Gc[(size_t)(i + k * N) * n + r];
-        if (r == i + k * N) g4[k].re -= 1.0;
+        g4[k] = Gc[(size_t)(site + k * N) * n + r];
+        if (r == site + k * N) g4[k].re -= 1.0;
       }
       for (int k = 0; k < 4; ++k) {
         cplx acc = mk(0, 0);
@@ -285,7 +343,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
     for (int jl = tid; jl < ncol; jl += blockDim.x) {
       const int j = cb0 + jl;
       cplx g4[4];
-      for (int k = 0; k < 4; ++k) g4[k] = Gc[(size_t)j * n + (i + k * N)];
+      for (int k = 0; k < 4; ++k) g4[k] = Gc[(size_t)j * n + (site + k * N)];
       for (int k = 0; k < 4; ++k) {
         cplx acc = mk(0, 0);
         for (int m = 0; m < 4; ++m) cfma(acc, sh.delta[k * 4 + m], g4[m]);
@@ -312,6 +370,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
     }
   }
   if (w == 0) {
+    __syncthreads();
     for (int e = tid; e < 3 * N; e += blockDim.x) c.phi[(size_t)3 * N * l + e] = phl[e];
     if (tid == 0) {
       *c.ucursor = cur;
@@ -330,7 +389,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
                                 const int* slices, double box, int wg, int log) {
   const int Nmax = nmax / 4;
-  const size_t lds = (size_t)(13 * Nmax + 2 * Nmax + 1) * 8 + (size_t)4 * nmax * 16 +
+  const size_t lds = (size_t)(17 * Nmax + 2 * Nmax + 1) * 8 + (size_t)4 * nmax * 16 +
                      (size_t)4 * ((nmax + wg - 1) / wg + 1) * 16;
   if (lds > 150 * 1024) return hipErrorInvalidValue;
   static size_t attr = 0;
