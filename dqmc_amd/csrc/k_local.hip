@@ -141,16 +141,28 @@ struct LocalShared {
   int stop;  // the uniform stream ran out
 };
 
-// entry e (row-major) of the site block [[C,S,0,R],[S*,C,-R,0],[0,-R,C,S*],[R,0,S,C]]
-__device__ inline cplx ev_entry(double C, cplx S, double R, int e) {
-  switch (e) {
-    case 0: case 5: case 10: case 15: return mk(C, 0);
-    case 1: case 14: return S;
-    case 4: case 11: return conj(S);
-    case 3: case 12: return mk(R, 0);
-    case 6: case 9: return mk(-R, 0);
-    default: return mk(0, 0);
-  }
+// Site block [[C,S,0,R],[S*,C,-R,0],[0,-R,C,S*],[R,0,S,C]] (interactions.jl:102-140): type of the
+// entry at row-major position e: 0 zero, 1 C, 2 S, 3 conj(S), 4 R, 5 -R.
+__device__ inline int ev_type(int e) {
+  const unsigned long long code = 0x1204315005134021ull;  // nibble e = type of entry e (e = 0 lowest)
+  return (int)((code >> (4 * e)) & 0xF);
+}
+__device__ inline cplx ev_entry_t(double C, cplx S, double R, int t) {
+  const double re = t == 1 ? C : (t == 2 || t == 3) ? S.re : t == 4 ? R : t == 5 ? -R : 0.0;
+  const double im = t == 2 ? S.im : t == 3 ? -S.im : 0.0;
+  return mk(re, im);
+}
+
+// determinant of a 4x4 by Laplace expansion along rows (0, 1) and (2, 3)
+__device__ inline cplx det4_laplace(const cplx* m) {
+  auto m2 = [&](int r, int p, int q) { return m[r * 4 + p] * m[(r + 1) * 4 + q] - m[r * 4 + q] * m[(r + 1) * 4 + p]; };
+  cplx d = m2(0, 0, 1) * m2(2, 2, 3);
+  d = d - m2(0, 0, 2) * m2(2, 1, 3);
+  d = d + m2(0, 0, 3) * m2(2, 1, 2);
+  d = d + m2(0, 1, 2) * m2(2, 0, 3);
+  d = d - m2(0, 1, 3) * m2(2, 0, 2);
+  d = d + m2(0, 2, 3) * m2(2, 0, 1);
+  return d;
 }
 
 __device__ inline cplx shfl_c(cplx v, int src) {
@@ -210,6 +222,15 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
 
   const int lane = tid & 63;
   const int ea = (lane & 15) >> 2, eb = lane & 3;  // this lane's entry of the 4x4 algebra
+  int t1[4], t2[4];  // site-block entry types of row ea (old operator) and column eb (new operator)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    t1[k] = ev_type(ea * 4 + k);
+    t2[k] = ev_type(k * 4 + eb);
+  }
+  const double k_time = 1.0 / (c.dtau * c.c * c.c), k_space = 0.5 * c.dtau, k_dt = c.dtau;
+  const double k_r = 0.5 * c.r, k_u = 0.25 * c.u;
+  const int edrun = c.edrun;
   long long cur = cur0;
   long long logi = dolog ? *c.log_count : 0;
   double dS_acc = 0.0;
@@ -252,12 +273,12 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
           ds_dot += snb * diff;
         }
         double dS;
-        if (!c.edrun) {
-          dS = 1.0 / (c.dtau * c.c * c.c) * (sq_diff - dt_dot);
-          dS += 0.5 * c.dtau * (4 * sq_diff - 2.0 * ds_dot);
-          dS += c.dtau * (0.5 * c.r * sq_diff + 0.25 * c.u * pow4_diff);
+        if (!edrun) {
+          dS = k_time * (sq_diff - dt_dot);
+          dS += k_space * (4 * sq_diff - 2.0 * ds_dot);
+          dS += k_dt * (k_r * sq_diff + k_u * pow4_diff);
         } else {
-          dS = c.dtau * (0.5 * c.r * sq_diff);
+          dS = k_dt * (k_r * sq_diff);
         }
         const double exp_dS = exp(-dS);
         // calc_detratio (local_updates.jl:42-59): delta = eV_op(old, -1) eV_op(new, +1) - I,
@@ -269,7 +290,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
         const cplx S1 = mk(e1c[4 * i + 1], e1c[4 * i + 2]);
         cplx dm = mk(0, 0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) cfma(dm, ev_entry(C1, S1, R1, ea * 4 + k), ev_entry(C2, S2, R2, k * 4 + eb));
+        for (int k = 0; k < 4; ++k) cfma(dm, ev_entry_t(C1, S1, R1, t1[k]), ev_entry_t(C2, S2, R2, t2[k]));
         if (ea == eb) dm.re -= 1.0;
         const cplx gm = mk((ea == eb ? 1.0 : 0.0) - gv.re, -gv.im);
         cplx mm = mk(0, 0);
@@ -279,7 +300,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
         cplx Mall[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) Mall[q] = shfl_c(mm, q);
-        const cplx det = det4(Mall);
+        const cplx det = det4_laplace(Mall);
         if (fabs(det.im / det.re) > 1e-4) ++nonreal;
         const double p_acc = exp_dS * det.re;
         int acc;
@@ -306,11 +327,19 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
           dS_acc += -log(exp_dS);
           if (lane == 0)
             for (int a = 0; a < 3; ++a) phl[3 * i + a] = nw[a];
-          if (lane < 16) sh.delta[lane] = dm;
-          cplx Mi[16];
-          inv4(Mall, Mi);
-          if (lane == 0)
-            for (int q = 0; q < 16; ++q) sh.minv[q] = Mi[q];
+          // M^-1 (b, a) = cofactor (a, b) / det M, one entry per lane
+          cplx m3[9];
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) m3[r * 3 + q] = shfl_c(mm, (r + (r >= ea)) * 4 + q + (q >= eb));
+          cplx cof = m3[0] * (m3[4] * m3[8] - m3[5] * m3[7]) - m3[1] * (m3[3] * m3[8] - m3[5] * m3[6]) +
+                     m3[2] * (m3[3] * m3[7] - m3[4] * m3[6]);
+          if ((ea + eb) & 1) cof = -cof;
+          if (lane < 16) {
+            sh.delta[lane] = dm;
+            sh.minv[eb * 4 + ea] = cof * cinv(det);
+          }
           site = i;
           ++i;
           break;

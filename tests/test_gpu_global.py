@@ -75,7 +75,7 @@ def test_global_updates_match_oracle(amd):
     assert st.global_proposed == len(oglob) == 2
     assert st.global_accepted == sum(oglob)
     assert _compare_logs(c.proposal_log(), olog) == 0
-    assert st.uniforms_used == ostream.pos
+    assert c.stream.consumed == ostream.pos
     assert np.abs(c.hsfield - o.phi).max() < 1e-12
     assert np.abs(c.greens - o.greens).max() < 1e-10
     assert abs(st.boson_action - o.boson_action) < 1e-9 * abs(o.boson_action)
