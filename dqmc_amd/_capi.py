@@ -51,6 +51,7 @@ SIGNATURES = {
     "dqmc_ctx_destroy": [_P],
     "dqmc_ctx_set_groups": [_P, _i, _i, _i],
     "dqmc_set_greens_route": [_P, _i],
+    "dqmc_set_local_variant": [_P, _i],
     "dqmc_set_field": [_P, _D],
     "dqmc_get_field": [_P, _D],
     "dqmc_set_uniforms": [_P, _D, _ll],

@@ -80,6 +80,8 @@ struct ChainDesc {
   int* inv_piv;           // [n]
   cplx* inv_bcast;        // [2][n]
   cplx* lu_small;         // [32] delta (16) and M^-1 (16) of the last single proposal
+  cplx* lu_win;           // [2][4 sites][4][n] columns published between local-update rounds
+  cplx* lu_diag;          // [2][N][16] diagonal site blocks published between rounds
   // local updates
   const double* uniforms; // host-drawn uniform stream (device copy)
   long long* ucursor;     // next unread uniform
@@ -196,9 +198,11 @@ hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, i
 // X <- X^-1 in place (Gauss-Jordan, partial pivoting); scal[S_LOGDET_A] = log|det X|.
 hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);
 
-// One local-update pass over all sites at `slice` (0-based).
+// One local-update pass over all sites at `slice` (0-based). tile = 1 keeps each workgroup's
+// column block of G in LDS (needs local_tile_fits), 0 updates G in global memory.
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
-                                const int* slices, double box, int wg, int log);
+                                const int* slices, double box, int wg, int log, int tile);
+bool local_tile_fits(int nmax, int wg);
 
 // scal[S_LOGDET] = log|det UG| + sum log DG + log|det TG|  helper pieces
 hipError_t launch_sum_log(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef d, int slot,

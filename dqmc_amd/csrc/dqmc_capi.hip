@@ -76,6 +76,7 @@ struct Group {
   int cur = 0, dir = 1;  // reference current_slice (1-based), direction
   int wg_qr = 1, wg_inv = 1, wg_local = 1;
   int greens_route = 0;  // 0: D-scale separation, 1: verbatim reference route
+  int local_variant = 0;  // 0: LDS-resident G when it fits, 1: G in global memory, 2: LDS-resident G
   bool recomputed = false;
   // kernel-class timing (dqmc_profile)
   unsigned prof_mask = 0;
@@ -208,6 +209,8 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
   ALLOC(d.inv_piv, int, n);
   ALLOC(d.inv_bcast, cplx, 2 * n);
   ALLOC(d.lu_small, cplx, 32);
+  ALLOC(d.lu_win, cplx, (size_t)2 * 4 * 4 * n);
+  ALLOC(d.lu_diag, cplx, (size_t)2 * 16 * N);
   ALLOC(d.ucursor, long long, 1);
   ALLOC(d.scal, double, kNumScalar);
   ALLOC(d.log_count, long long, 1);
@@ -819,7 +822,10 @@ struct Ops {
   int local_updates(double box, int dolog) {
     if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "local_updates outside slices 1..M");
     ProfScope ps(g, DQMC_K_LOCAL);
-    LCHK(launch_local_updates(s(), d(), nc(), g.n, g.cur - 1, nullptr, box, g.wg_local, dolog));
+    const bool fits = local_tile_fits(g.n, g.wg_local);
+    if (g.local_variant == 2 && !fits) return fail(DQMC_E_ARG, "G tiles do not fit in LDS at this workgroup count");
+    const int tile = g.local_variant == 2 || (g.local_variant == 0 && fits);
+    LCHK(launch_local_updates(s(), d(), nc(), g.n, g.cur - 1, nullptr, box, g.wg_local, dolog, tile));
     return 0;
   }
 };
@@ -1074,6 +1080,13 @@ int dqmc_set_greens_route(dqmc_ctx* ctx, int route) {
   CTX_CHECK(ctx);
   if (route != 0 && route != 1) return fail(DQMC_E_ARG, "route must be 0 or 1");
   ctx->b->groups[0].greens_route = route;
+  return 0;
+}
+
+int dqmc_set_local_variant(dqmc_ctx* ctx, int variant) {
+  CTX_CHECK(ctx);
+  if (variant < 0 || variant > 2) return fail(DQMC_E_ARG, "local variant must be 0, 1 or 2");
+  for (auto& g : ctx->b->groups) g.local_variant = variant;
   return 0;
 }
 

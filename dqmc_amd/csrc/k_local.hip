@@ -169,183 +169,248 @@ __device__ inline cplx shfl_c(cplx v, int src) {
   return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64));
 }
 
-// Proposals are evaluated by wave 0 alone, every lane holding one entry (a, b) of the 4x4
-// algebra (lanes 16..63 mirror 0..15); the other waves wait at one workgroup barrier per
-// accepted proposal, so a rejected proposal synchronises nothing.
-__global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, int slice, const int* slices,
-                                                       double box, int dolog) {
-  const ChainDesc& c = chains[blockIdx.y];
-  const int n = c.n, N = c.N, M = c.M;
-  const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
-  const int l = slices ? slices[blockIdx.y] : slice;
-  GroupSync g = chain_group(c);
-  // own column block [cb0, cb1)
-  const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
-  const int ncol = cb1 - cb0;
+// ---------------------------------------------------------------------------
+// Shared pieces of both local-update kernels
+// ---------------------------------------------------------------------------
 
-  extern __shared__ double smem[];
-  double* phl = smem;              // [3N] current slice
-  double* phe = phl + 3 * N;       // [3N] earlier slice
-  double* pla = phe + 3 * N;       // [3N] later slice
-  int* nbl = reinterpret_cast<int*>(pla + 3 * N);  // [4N]
-  double* ul = smem + 9 * N + 2 * N;               // [4N] this slice's uniforms
-  double* e1c = ul + 4 * N;                        // [4N] C, S.re, S.im, R of eV_op(phi_i, -1)
-  cplx* Acol = reinterpret_cast<cplx*>(smem + 17 * N + 2 * N + (N & 1));  // [n][4]
-  cplx* Brow = Acol + 4 * (size_t)n;                                       // [4][ncol]
-  __shared__ LocalShared sh;
+// LDS view of one slice: field of this and the two neighbouring slices, neighbour
+// table, the slice's uniforms and the old-field site blocks eV_op(phi_i, -1).
+struct SliceLds {
+  double* phl;  // [3N] current slice
+  double* phe;  // [3N] earlier slice
+  double* pla;  // [3N] later slice
+  int* nbl;     // [4N]
+  double* ul;   // [4N]
+  double* e1c;  // [4N] C, S.re, S.im, R
+  long long cur0, navail;
+};
+// doubles of LDS SliceLds takes (rounded up to keep 16-byte alignment after it)
+__host__ __device__ inline size_t slice_lds_doubles(int N) { return (size_t)19 * N + (N & 1); }
 
-  const double lam_dtau = c.lam * c.dtau;
+__device__ inline SliceLds load_slice(const ChainDesc& c, int l, double* smem) {
+  const int N = c.N, M = c.M, tid = threadIdx.x;
+  SliceLds s;
+  s.phl = smem;
+  s.phe = s.phl + 3 * N;
+  s.pla = s.phe + 3 * N;
+  s.nbl = reinterpret_cast<int*>(s.pla + 3 * N);
+  s.ul = smem + 11 * N;
+  s.e1c = s.ul + 4 * N;
   const int le = (l == 0) ? M - 1 : l - 1;  // time_neighbors (lattice.jl:144-153)
   const int la = (l == M - 1) ? 0 : l + 1;
   for (int e = tid; e < 3 * N; e += blockDim.x) {
-    phl[e] = c.phi[(size_t)3 * N * l + e];
-    phe[e] = c.phi[(size_t)3 * N * le + e];
-    pla[e] = c.phi[(size_t)3 * N * la + e];
+    s.phl[e] = c.phi[(size_t)3 * N * l + e];
+    s.phe[e] = c.phi[(size_t)3 * N * le + e];
+    s.pla[e] = c.phi[(size_t)3 * N * la + e];
   }
-  for (int e = tid; e < 4 * N; e += blockDim.x) nbl[e] = c.nb[e];
+  for (int e = tid; e < 4 * N; e += blockDim.x) s.nbl[e] = c.nb[e];
   // at most 4 draws per proposal: this slice consumes no more than 4N uniforms
-  const long long cur0 = *c.ucursor;
-  const long long navail = min((long long)4 * N, c.n_uniforms - cur0);
-  for (long long e = tid; e < navail; e += blockDim.x) ul[e] = c.uniforms[cur0 + e];
+  s.cur0 = *c.ucursor;
+  s.navail = min((long long)4 * N, c.n_uniforms - s.cur0);
+  for (long long e = tid; e < s.navail; e += blockDim.x) s.ul[e] = c.uniforms[s.cur0 + e];
   // each site is visited once per slice, so its old field is the value at kernel start
+  const double lam_dtau = c.lam * c.dtau;
   for (int i = tid; i < N; i += blockDim.x) {
     const double* f = c.phi + (size_t)3 * N * l + 3 * i;
     double C, R;
     cplx S;
     ev_coeffs_op(lam_dtau, f[0], f[1], f[2], -1.0, C, S, R);
-    e1c[4 * i + 0] = C;
-    e1c[4 * i + 1] = S.re;
-    e1c[4 * i + 2] = S.im;
-    e1c[4 * i + 3] = R;
+    s.e1c[4 * i + 0] = C;
+    s.e1c[4 * i + 1] = S.re;
+    s.e1c[4 * i + 2] = S.im;
+    s.e1c[4 * i + 3] = R;
   }
-  __syncthreads();
+  return s;
+}
 
-  const int lane = tid & 63;
-  const int ea = (lane & 15) >> 2, eb = lane & 3;  // this lane's entry of the 4x4 algebra
+// Wave-0 state of the proposal sequence of one slice.
+struct Proposals {
+  long long cur, logi;
+  double dS_acc, minratio;
+  int nacc, nonreal, i;
+};
+
+// Evaluate proposals from site p.i on, in wave 0 (every lane holds entry (ea, eb) of the 4x4
+// algebra; lanes 16..63 mirror 0..15), until one is accepted: its delta and M^-1 go to
+// sh.delta / sh.minv, the field of the site is updated in LDS and the site is returned.
+// Returns N at the end of the slice; sets *stop when the uniform stream ran out.
+// diag(j) returns this lane's entry G_jj(ea, eb) of the current G.
+// Reference: local_updates (local_updates.jl:1-39), calc_detratio (:42-59),
+// calc_boson_action_diff (action.jl:57-101).
+template <class Diag>
+__device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposals& p, LocalShared& sh,
+                                  double box, int dolog, bool logger, Diag diag, int* stop) {
+  const int N = c.N, lane = threadIdx.x & 63;
+  const int ea = (lane & 15) >> 2, eb = lane & 3;
   int t1[4], t2[4];  // site-block entry types of row ea (old operator) and column eb (new operator)
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     t1[k] = ev_type(ea * 4 + k);
     t2[k] = ev_type(k * 4 + eb);
   }
+  const double lam_dtau = c.lam * c.dtau;
   const double k_time = 1.0 / (c.dtau * c.c * c.c), k_space = 0.5 * c.dtau, k_dt = c.dtau;
   const double k_r = 0.5 * c.r, k_u = 0.25 * c.u;
   const int edrun = c.edrun;
-  long long cur = cur0;
-  long long logi = dolog ? *c.log_count : 0;
-  double dS_acc = 0.0;
-  int nacc = 0, nonreal = 0;
-  double minratio = c.scal[S_MINRATIO];
-  int buf = 0, i = 0;
-  cplx* Gb[2] = {c.mat[M_G], c.mat[M_GALT]};
+  *stop = 0;
+  int i = p.i;
+  cplx gnext = mk(0, 0);  // G_ii of the next site, loaded one site ahead
+  if (i < N) gnext = diag(i);
+  for (; i < N; ++i) {
+    const cplx gv = gnext;
+    if (i + 1 < N) gnext = diag(i + 1);
+    if ((p.cur - s.cur0) + 4 > s.navail) {
+      *stop = 1;
+      break;
+    }
+    const double* u = s.ul + (p.cur - s.cur0);
+    double old[3], nw[3];
+    for (int a = 0; a < 3; ++a) {
+      old[a] = s.phl[3 * i + a];
+      nw[a] = old[a] + (-box + 2.0 * box * u[a]);
+    }
+    const double old_sq = old[0] * old[0] + old[1] * old[1] + old[2] * old[2];
+    const double new_sq = nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2];
+    const double sq_diff = new_sq - old_sq;
+    const double pow4_diff = new_sq * new_sq - old_sq * old_sq;
+    double dt_dot = 0.0, ds_dot = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double diff = nw[a] - old[a];
+      const double tnb = s.pla[3 * i + a] + s.phe[3 * i + a];
+      double snb = 0.0;
+      for (int k = 0; k < 4; ++k) snb += s.phl[3 * s.nbl[k * N + i] + a];
+      dt_dot += tnb * diff;
+      ds_dot += snb * diff;
+    }
+    double dS;
+    if (!edrun) {
+      dS = k_time * (sq_diff - dt_dot);
+      dS += k_space * (4 * sq_diff - 2.0 * ds_dot);
+      dS += k_dt * (k_r * sq_diff + k_u * pow4_diff);
+    } else {
+      dS = k_dt * (k_r * sq_diff);
+    }
+    const double exp_dS = exp(-dS);
+    // delta = eV_op(old, -1) eV_op(new, +1) - I,  M = I + delta (I - G_ii)
+    double C2, R2;
+    cplx S2;
+    ev_coeffs_op(lam_dtau, nw[0], nw[1], nw[2], 1.0, C2, S2, R2);
+    const double C1 = s.e1c[4 * i], R1 = s.e1c[4 * i + 3];
+    const cplx S1 = mk(s.e1c[4 * i + 1], s.e1c[4 * i + 2]);
+    cplx dm = mk(0, 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cfma(dm, ev_entry_t(C1, S1, R1, t1[k]), ev_entry_t(C2, S2, R2, t2[k]));
+    if (ea == eb) dm.re -= 1.0;
+    const cplx gm = mk((ea == eb ? 1.0 : 0.0) - gv.re, -gv.im);
+    cplx mm = mk(0, 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cfma(mm, shfl_c(dm, ea * 4 + k), shfl_c(gm, k * 4 + eb));
+    if (ea == eb) mm.re += 1.0;
+    cplx Mall[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Mall[q] = shfl_c(mm, q);
+    const cplx det = det4_laplace(Mall);
+    if (fabs(det.im / det.re) > 1e-4) ++p.nonreal;
+    const double p_acc = exp_dS * det.re;
+    int acc;
+    double uval = -1.0;
+    if (p_acc > 1.0) {
+      acc = 1;
+      p.cur += 3;
+    } else {
+      uval = u[3];
+      acc = uval < p_acc;
+      p.cur += 4;
+      p.minratio = fmin(p.minratio, fabs(p_acc - uval));
+    }
+    if (dolog && logger && lane == 0 && p.logi < c.log_cap) {
+      double* lg = c.prop_log + 4 * p.logi;
+      lg[0] = p_acc;
+      lg[1] = uval;
+      lg[2] = acc;
+      lg[3] = det.im / det.re;
+    }
+    ++p.logi;
+    if (acc) {
+      ++p.nacc;
+      p.dS_acc += -log(exp_dS);
+      if (lane == 0)
+        for (int a = 0; a < 3; ++a) s.phl[3 * i + a] = nw[a];
+      // M^-1 (b, a) = cofactor (a, b) / det M, one entry per lane
+      cplx m3[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) m3[r * 3 + q] = shfl_c(mm, (r + (r >= ea)) * 4 + q + (q >= eb));
+      cplx cof = m3[0] * (m3[4] * m3[8] - m3[5] * m3[7]) - m3[1] * (m3[3] * m3[8] - m3[5] * m3[6]) +
+                 m3[2] * (m3[3] * m3[7] - m3[4] * m3[6]);
+      if ((ea + eb) & 1) cof = -cof;
+      if (lane < 16) {
+        sh.delta[lane] = dm;
+        sh.minv[eb * 4 + ea] = cof * cinv(det);
+      }
+      p.i = i + 1;
+      return i;
+    }
+  }
+  p.i = i;
+  return N;
+}
 
+// counters and the updated field slice, written by workgroup 0
+__device__ inline void finish_slice(const ChainDesc& c, int l, const SliceLds& s, const Proposals& p, int dolog,
+                                    int stop) {
+  const int N = c.N, tid = threadIdx.x;
+  __syncthreads();
+  for (int e = tid; e < 3 * N; e += blockDim.x) c.phi[(size_t)3 * N * l + e] = s.phl[e];
+  if (tid == 0) {
+    *c.ucursor = p.cur;
+    if (dolog) *c.log_count = p.logi;
+    c.scal[S_BOSON_ACTION] += p.dS_acc;
+    c.scal[S_ACC_SLICE] = p.nacc;
+    c.scal[S_ACC_TOTAL] += p.nacc;
+    c.scal[S_PROP_TOTAL] += N;
+    c.scal[S_NONREAL] += p.nonreal;
+    c.scal[S_MINRATIO] = p.minratio;
+    if (stop) *c.error = 2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Variant 1: G in global memory, ping-pong buffers (any size).
+// Every workgroup evaluates every proposal redundantly; on acceptance each
+// workgroup reads the site's four columns, updates its own column block into the
+// partner buffer; one group barrier per accepted proposal.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, int slice, const int* slices,
+                                                       double box, int dolog) {
+  const ChainDesc& c = chains[blockIdx.y];
+  const int n = c.n, N = c.N;
+  const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+  const int l = slices ? slices[blockIdx.y] : slice;
+  GroupSync g = chain_group(c);
+  const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
+  const int ncol = cb1 - cb0;
+
+  extern __shared__ double smem[];
+  const SliceLds s = load_slice(c, l, smem);
+  cplx* Acol = reinterpret_cast<cplx*>(smem + slice_lds_doubles(N));  // [n][4]
+  cplx* Brow = Acol + 4 * (size_t)n;                                  // [4][ncol]
+  __shared__ LocalShared sh;
+  __syncthreads();
+
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  int buf = 0;
+  cplx* Gb[2] = {c.mat[M_G], c.mat[M_GALT]};
   for (;;) {
     if (tid < 64) {
-      int site = N, stop = 0;
-      // G_ii(a, b) of the site under evaluation, loaded one site ahead (valid while G is unchanged)
-      cplx gnext = mk(0, 0);
-      if (i < N) gnext = Gb[buf][(size_t)(i + eb * N) * n + (i + ea * N)];
-      for (; i < N; ++i) {
-        const cplx gv = gnext;
-        if (i + 1 < N) gnext = Gb[buf][(size_t)(i + 1 + eb * N) * n + (i + 1 + ea * N)];
-        if ((cur - cur0) + 4 > navail) {
-          stop = 1;
-          break;
-        }
-        const double* u = ul + (cur - cur0);
-        double old[3], nw[3];
-        for (int a = 0; a < 3; ++a) {
-          old[a] = phl[3 * i + a];
-          nw[a] = old[a] + (-box + 2.0 * box * u[a]);
-        }
-        // calc_boson_action_diff (action.jl:57-101)
-        const double old_sq = old[0] * old[0] + old[1] * old[1] + old[2] * old[2];
-        const double new_sq = nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2];
-        const double sq_diff = new_sq - old_sq;
-        const double pow4_diff = new_sq * new_sq - old_sq * old_sq;
-        double dt_dot = 0.0, ds_dot = 0.0;
-        for (int a = 0; a < 3; ++a) {
-          const double diff = nw[a] - old[a];
-          const double tnb = pla[3 * i + a] + phe[3 * i + a];
-          double snb = 0.0;
-          for (int k = 0; k < 4; ++k) snb += phl[3 * nbl[k * N + i] + a];
-          dt_dot += tnb * diff;
-          ds_dot += snb * diff;
-        }
-        double dS;
-        if (!edrun) {
-          dS = k_time * (sq_diff - dt_dot);
-          dS += k_space * (4 * sq_diff - 2.0 * ds_dot);
-          dS += k_dt * (k_r * sq_diff + k_u * pow4_diff);
-        } else {
-          dS = k_dt * (k_r * sq_diff);
-        }
-        const double exp_dS = exp(-dS);
-        // calc_detratio (local_updates.jl:42-59): delta = eV_op(old, -1) eV_op(new, +1) - I,
-        // M = I + delta (I - G_ii)
-        double C2, R2;
-        cplx S2;
-        ev_coeffs_op(lam_dtau, nw[0], nw[1], nw[2], 1.0, C2, S2, R2);
-        const double C1 = e1c[4 * i], R1 = e1c[4 * i + 3];
-        const cplx S1 = mk(e1c[4 * i + 1], e1c[4 * i + 2]);
-        cplx dm = mk(0, 0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cfma(dm, ev_entry_t(C1, S1, R1, t1[k]), ev_entry_t(C2, S2, R2, t2[k]));
-        if (ea == eb) dm.re -= 1.0;
-        const cplx gm = mk((ea == eb ? 1.0 : 0.0) - gv.re, -gv.im);
-        cplx mm = mk(0, 0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cfma(mm, shfl_c(dm, ea * 4 + k), shfl_c(gm, k * 4 + eb));
-        if (ea == eb) mm.re += 1.0;
-        cplx Mall[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) Mall[q] = shfl_c(mm, q);
-        const cplx det = det4_laplace(Mall);
-        if (fabs(det.im / det.re) > 1e-4) ++nonreal;
-        const double p_acc = exp_dS * det.re;
-        int acc;
-        double uval = -1.0;
-        if (p_acc > 1.0) {
-          acc = 1;
-          cur += 3;
-        } else {
-          uval = u[3];
-          acc = uval < p_acc;
-          cur += 4;
-          minratio = fmin(minratio, fabs(p_acc - uval));
-        }
-        if (dolog && w == 0 && lane == 0 && logi < c.log_cap) {
-          double* lg = c.prop_log + 4 * logi;
-          lg[0] = p_acc;
-          lg[1] = uval;
-          lg[2] = acc;
-          lg[3] = det.im / det.re;
-        }
-        ++logi;
-        if (acc) {
-          ++nacc;
-          dS_acc += -log(exp_dS);
-          if (lane == 0)
-            for (int a = 0; a < 3; ++a) phl[3 * i + a] = nw[a];
-          // M^-1 (b, a) = cofactor (a, b) / det M, one entry per lane
-          cplx m3[9];
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) m3[r * 3 + q] = shfl_c(mm, (r + (r >= ea)) * 4 + q + (q >= eb));
-          cplx cof = m3[0] * (m3[4] * m3[8] - m3[5] * m3[7]) - m3[1] * (m3[3] * m3[8] - m3[5] * m3[6]) +
-                     m3[2] * (m3[3] * m3[7] - m3[4] * m3[6]);
-          if ((ea + eb) & 1) cof = -cof;
-          if (lane < 16) {
-            sh.delta[lane] = dm;
-            sh.minv[eb * 4 + ea] = cof * cinv(det);
-          }
-          site = i;
-          ++i;
-          break;
-        }
-      }
-      if (lane == 0) {
+      int stop;
+      const cplx* Gcur = Gb[buf];
+      const int lane = tid & 63, ea = (lane & 15) >> 2, eb = lane & 3;
+      const int site = evaluate_proposals(
+          c, s, p, sh, box, dolog, w == 0, [&](int j) { return Gcur[(size_t)(j + eb * N) * n + (j + ea * N)]; },
+          &stop);
+      if (tid == 0) {
         sh.site = site;
         sh.stop = stop;
       }
@@ -398,38 +463,166 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
       Gb[0][o] = Gb[1][o];
     }
   }
-  if (w == 0) {
-    __syncthreads();
-    for (int e = tid; e < 3 * N; e += blockDim.x) c.phi[(size_t)3 * N * l + e] = phl[e];
-    if (tid == 0) {
-      *c.ucursor = cur;
-      if (dolog) *c.log_count = logi;
-      c.scal[S_BOSON_ACTION] += dS_acc;
-      c.scal[S_ACC_SLICE] = nacc;
-      c.scal[S_ACC_TOTAL] += nacc;
-      c.scal[S_PROP_TOTAL] += N;
-      c.scal[S_NONREAL] += nonreal;
-      c.scal[S_MINRATIO] = minratio;
-      if (sh.stop) *c.error = 2;
-    }
-  }
+  if (w == 0) finish_slice(c, l, s, p, dolog, sh.stop);
 }
 
+// ---------------------------------------------------------------------------
+// Variant 2: G resident in LDS. Workgroup w holds its column block of G in LDS for
+// the whole slice and applies every accepted rank-4 update there. What the other
+// workgroups need crosses global memory in "rounds" (publish, then one group
+// barrier), double-buffered by round parity:
+//   * the 4x4 diagonal blocks G_jj of every site not yet visited (decisions), and
+//   * the full columns of the next kLocalWindow sites (the rank-4 left factor of
+//     the next accepted proposal; a proposal accepted beyond the window costs one
+//     extra round that publishes a window starting at it).
+// Before the first round of a slice both come from G in global memory, which the
+// tiles were loaded from. Global G is written back once, at the end.
+// ---------------------------------------------------------------------------
+constexpr int kLocalWindow = 4;
+
+__global__ __launch_bounds__(256) void k_local_updates_tile(const ChainDesc* chains, int slice, const int* slices,
+                                                            double box, int dolog) {
+  const ChainDesc& c = chains[blockIdx.y];
+  const int n = c.n, N = c.N;
+  const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+  const int l = slices ? slices[blockIdx.y] : slice;
+  GroupSync g = chain_group(c);
+  const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
+  const int ncol = cb1 - cb0;
+
+  extern __shared__ double smem[];
+  const SliceLds s = load_slice(c, l, smem);
+  cplx* T = reinterpret_cast<cplx*>(smem + slice_lds_doubles(N));  // [ncol][n] own columns of G
+  cplx* Brow = T + (size_t)ncol * n;                                // [4][ncol]
+  __shared__ LocalShared sh;
+  __shared__ int pub_site;  // first site of the window published in the last round
+  const cplx* G0 = c.mat[M_G];
+  for (size_t e = tid; e < (size_t)ncol * n; e += blockDim.x) T[e] = G0[(size_t)cb0 * n + e];
+  if (tid == 0) pub_site = -1;
+  __syncthreads();
+
+  const size_t win_sz = (size_t)kLocalWindow * 4 * n;  // complex entries of one window
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  int rnd = 0;  // completed rounds; buffers of round r live at parity r & 1
+  int stop = 0;
+
+  // one round: publish the diagonal blocks of sites >= from and the columns of sites
+  // win0 .. win0 + kLocalWindow - 1 from the own tile, then a group barrier
+  auto publish_round = [&](int from, int win0) -> bool {
+    const int par = (rnd + 1) & 1;
+    cplx* dg = c.lu_diag + (size_t)par * 16 * N;
+    cplx* wn = c.lu_win + (size_t)par * win_sz;
+    for (int jl = tid; jl < ncol; jl += blockDim.x) {
+      const int col = cb0 + jl, j = col % N, b = col / N;
+      if (j >= from)
+        for (int a = 0; a < 4; ++a) dg[16 * j + 4 * a + b] = T[(size_t)jl * n + j + a * N];
+    }
+    for (int q = 0; q < kLocalWindow * 4; ++q) {
+      const int site = win0 + (q >> 2);
+      if (site >= N) break;
+      const int col = site + (q & 3) * N;
+      if (col < cb0 || col >= cb1) continue;
+      const cplx* src = T + (size_t)(col - cb0) * n;
+      cplx* dst = wn + (size_t)q * n;
+      for (int r = tid; r < n; r += blockDim.x) dst[r] = src[r];
+    }
+    ++rnd;
+    if (tid == 0) pub_site = win0;
+    group_sync(g);
+    return !group_failed(g);
+  };
+
+  for (;;) {
+    if (tid < 64) {
+      const int lane = tid & 63, ea = (lane & 15) >> 2, eb = lane & 3;
+      const cplx* dg = c.lu_diag + (size_t)(rnd & 1) * 16 * N;
+      const bool from_g = rnd == 0;
+      int st;
+      const int site = evaluate_proposals(
+          c, s, p, sh, box, dolog, w == 0,
+          [&](int j) { return from_g ? G0[(size_t)(j + eb * N) * n + (j + ea * N)] : dg[16 * j + 4 * ea + eb]; },
+          &st);
+      if (tid == 0) {
+        sh.site = site;
+        sh.stop = st;
+      }
+    }
+    __syncthreads();
+    const int site = sh.site;
+    stop = sh.stop;
+    if (site >= N) break;
+    // the site's four columns of the current G: global G before the first round, else the window
+    if (rnd > 0 && !(site >= pub_site && site < pub_site + kLocalWindow)) {
+      if (!publish_round(site, site)) return;
+    }
+    const cplx* colsrc[4];
+    for (int k = 0; k < 4; ++k)
+      colsrc[k] = rnd == 0 ? G0 + (size_t)(site + k * N) * n
+                           : c.lu_win + (size_t)(rnd & 1) * win_sz + (size_t)((site - pub_site) * 4 + k) * n;
+    // B = delta G[r_i, own columns] (4 x ncol), from the tile
+    for (int jl = tid; jl < ncol; jl += blockDim.x) {
+      cplx g4[4];
+      for (int k = 0; k < 4; ++k) g4[k] = T[(size_t)jl * n + site + k * N];
+      for (int k = 0; k < 4; ++k) {
+        cplx acc = mk(0, 0);
+        for (int m = 0; m < 4; ++m) cfma(acc, sh.delta[k * 4 + m], g4[m]);
+        Brow[(size_t)k * ncol + jl] = acc;
+      }
+    }
+    __syncthreads();
+    // T[r, :] += A[r, :] B with A = (G[:, c_i] - E) M^-1 formed per row (update_greens!,
+    // local_updates.jl:61-95)
+    for (int r = tid; r < n; r += blockDim.x) {
+      cplx g4[4], a4[4];
+      for (int k = 0; k < 4; ++k) {
+        g4[k] = colsrc[k][r];
+        if (r == site + k * N) g4[k].re -= 1.0;
+      }
+      for (int k = 0; k < 4; ++k) {
+        cplx acc = mk(0, 0);
+        for (int m = 0; m < 4; ++m) cfma(acc, g4[m], sh.minv[m * 4 + k]);
+        a4[k] = acc;
+      }
+      for (int jl = 0; jl < ncol; ++jl) {
+        cplx acc = T[(size_t)jl * n + r];
+        for (int k = 0; k < 4; ++k) cfma(acc, a4[k], Brow[(size_t)k * ncol + jl]);
+        T[(size_t)jl * n + r] = acc;
+      }
+    }
+    __syncthreads();
+    if (!publish_round(site + 1, site + 1)) return;
+  }
+  cplx* Gout = c.mat[M_G];
+  for (size_t e = tid; e < (size_t)ncol * n; e += blockDim.x) Gout[(size_t)cb0 * n + e] = T[e];
+  if (w == 0) finish_slice(c, l, s, p, dolog, stop);
+}
+
+// LDS bytes of the two variants at the given shape (dynamic part)
+static size_t local_lds_global(int n, int N, int wg) {
+  return slice_lds_doubles(N) * 8 + (size_t)4 * n * 16 + (size_t)4 * ((n + wg - 1) / wg + 1) * 16;
+}
+static size_t local_lds_tile(int n, int N, int wg) {
+  const size_t ncol = (size_t)(n + wg - 1) / wg;
+  return slice_lds_doubles(N) * 8 + ncol * n * 16 + (size_t)4 * ncol * 16;
+}
+constexpr size_t kLocalLdsMax = 150 * 1024;
+
+bool local_tile_fits(int nmax, int wg) { return local_lds_tile(nmax, nmax / 4, wg) <= kLocalLdsMax; }
+
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
-                                const int* slices, double box, int wg, int log) {
+                                const int* slices, double box, int wg, int log, int tile) {
   const int Nmax = nmax / 4;
-  const size_t lds = (size_t)(17 * Nmax + 2 * Nmax + 1) * 8 + (size_t)4 * nmax * 16 +
-                     (size_t)4 * ((nmax + wg - 1) / wg + 1) * 16;
-  if (lds > 150 * 1024) return hipErrorInvalidValue;
-  static size_t attr = 0;
-  if (lds > 65536 && lds > attr) {
-    hipError_t e =
-        hipFuncSetAttribute((const void*)k_local_updates, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const void* fn = tile ? (const void*)k_local_updates_tile : (const void*)k_local_updates;
+  const size_t lds = tile ? local_lds_tile(nmax, Nmax, wg) : local_lds_global(nmax, Nmax, wg);
+  if (lds > kLocalLdsMax) return hipErrorInvalidValue;
+  static size_t attr[2] = {0, 0};
+  if (lds > 65536 && lds > attr[tile ? 1 : 0]) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    attr = lds;
+    attr[tile ? 1 : 0] = lds;
   }
   void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_local_updates, dim3(wg, nchains), dim3(256), args, lds, s);
+  hipError_t e = hipLaunchCooperativeKernel(fn, dim3(wg, nchains), dim3(256), args, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
   return e;
 }

@@ -199,6 +199,10 @@ class DQMC:
     def set_groups(self, wg_qr=0, wg_inv=0, wg_local=0):
         check(self.lib.dqmc_ctx_set_groups(self.h, wg_qr, wg_inv, wg_local))
 
+    def set_local_variant(self, variant):
+        """0: G tiles in LDS when they fit (default); 1: G in global memory; 2: LDS only."""
+        check(self.lib.dqmc_set_local_variant(self.h, variant))
+
     def set_greens_route(self, route):
         """0: D-scale separation (default); 1: the reference's calculate_greens verbatim."""
         check(self.lib.dqmc_set_greens_route(self.h, route))

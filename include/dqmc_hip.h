@@ -103,6 +103,11 @@ int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local);
  * inner UDT decompositions, myrdiv and inv). Both give the same G and log|det G|
  * to rounding; the setting applies to every later recompute of the context. */
 int dqmc_set_greens_route(dqmc_ctx* ctx, int route);
+/* where local_updates (local_updates.jl:1-39) keeps G while it sweeps the sites of a
+ * slice: 0 (default) each workgroup's column block in LDS when it fits, else global
+ * memory; 1 always global memory; 2 always LDS (error when it does not fit). Same
+ * results either way. */
+int dqmc_set_local_variant(dqmc_ctx* ctx, int variant);
 
 /* p.hsfield (parameters.jl:19): phi[a + 3*(i + N*l)] */
 int dqmc_set_field(dqmc_ctx* ctx, const double* phi);

@@ -265,11 +265,13 @@ def test_local_update_kats(amd):
     c.close()
 
 
-def _run_pair(mod, DQMC, p, seed, nsweeps, per_slice=False):
+def _run_pair(mod, DQMC, p, seed, nsweeps, per_slice=False, setup=None):
     rng = np.random.default_rng(seed)
     phi = rng.uniform(-1, 1, (3, p.N, p.slices))
     from dqmc_amd import UniformStream
     c = DQMC(p, uniforms=UniformStream(seed + 100))
+    if setup is not None:
+        setup(c)
     c.set_proposal_log(p.N * p.slices * nsweeps + 16)
     c.init(phi)
     o = O.DQMC(oracle_params(p), phi)
@@ -319,6 +321,26 @@ def test_seeded_chain_small(amd):
     assert _compare_logs(c2.proposal_log(), olog2) == 0
     assert np.abs(c2.greens - o2.greens).max() < 1e-10
     c2.close()
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_local_update_variants(amd, variant):
+    """G in global memory (1) and G tiles resident in LDS (2): same decisions as the oracle,
+    G within 1e-10, for several workgroup counts (L=6, beta=2)."""
+    mod, DQMC = amd
+    for wg in (3, 8, 36):
+        _say("local-update variant %d, %d workgroups, L=6 M=20, 3 sweeps" % (variant, wg))
+        p = config_params(mod, 6, 20)
+
+        def setup(c):
+            c.set_groups(0, 0, wg)
+            c.set_local_variant(variant)
+
+        c, o, olog, _ = _run_pair(mod, DQMC, p, 50 + wg, 3, setup=setup)
+        assert _compare_logs(c.proposal_log(), olog) == 0
+        assert np.abs(c.greens - o.greens).max() < 1e-10
+        assert c.stats().drift_max < 1e-8
+        c.close()
 
 
 def test_seeded_chain_dense(amd):
