@@ -52,6 +52,7 @@ SIGNATURES = {
     "dqmc_ctx_set_groups": [_P, _i, _i, _i],
     "dqmc_set_greens_route": [_P, _i],
     "dqmc_set_local_variant": [_P, _i],
+    "dqmc_set_udt_route": [_P, _i],
     "dqmc_set_field": [_P, _D],
     "dqmc_get_field": [_P, _D],
     "dqmc_set_uniforms": [_P, _D, _ll],
@@ -71,6 +72,7 @@ SIGNATURES = {
     "dqmc_apply_B": [_P, _i, _i, _D],
     "dqmc_wrap_greens": [_P, _D, _i, _i],
     "dqmc_udt": [_P, _D, _D, _D, _D],
+    "dqmc_udt_prepivoted": [_P, _D, _D, _D, _D],
     "dqmc_calculate_greens": [_P, _D, _D, _D, _D, _D, _D, _D, _D],
     "dqmc_greens_at": [_P, _i, _D, _D],
     "dqmc_detratio": [_P, _i, _D, _D, _D, _D],

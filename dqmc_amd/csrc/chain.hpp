@@ -77,6 +77,7 @@ struct ChainDesc {
   int* qr_perm;           // [n] logical -> physical column
   int* qr_lpos;           // [n] physical -> logical
   cplx* qr_tau;           // [n]
+  cplx* qr_Tf;            // [16 n] compact-WY T factors of the panels of the blocked QR
   int* inv_piv;           // [n]
   cplx* inv_bcast;        // [2][n]
   cplx* lu_small;         // [32] delta (16) and M^-1 (16) of the last single proposal
@@ -194,6 +195,13 @@ hipError_t launch_udt(hipStream_t s, const ChainDesc* chains, int nchains, int n
 hipError_t launch_qrp(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D, int wg);
 hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
                           MatRef Qout, MatRef Tout);
+
+// Blocked Householder QR of A (in place, k_qrp layout; D = |Re R_kk|); prepivot = 1 orders
+// the columns by decreasing initial norm first, 0 keeps them.
+hipError_t launch_qr_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
+                             int prepivot, int wg);
+// B <- R^-1 Q^H B with the factors of launch_qr_blocked(prepivot = 0) in A
+hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B);
 
 // X <- X^-1 in place (Gauss-Jordan, partial pivoting); scal[S_LOGDET_A] = log|det X|.
 hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);

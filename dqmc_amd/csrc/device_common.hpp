@@ -57,11 +57,30 @@ __host__ __device__ inline cplx cinv(cplx a) {
 
 // ---------------------------------------------------------------------------
 // Reductions (wave64)
+//
+// wave_sum: inclusive scans inside each row of 16 lanes with DPP row shifts (zero fill),
+// then the four row totals (lanes 15, 31, 47, 63) are read into scalar registers and
+// added. No LDS traffic; every lane returns the same value. Needs all 64 lanes active.
 // ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ inline double dpp_move(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+__device__ inline double read_lane(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0x111>(v);  // row_shr:1
+  v += dpp_move<0x112>(v);  // row_shr:2
+  v += dpp_move<0x114>(v);  // row_shr:4
+  v += dpp_move<0x118>(v);  // row_shr:8
+  return (read_lane(v, 15) + read_lane(v, 31)) + (read_lane(v, 47) + read_lane(v, 63));
 }
 __device__ inline cplx wave_sum(cplx v) {
   v.re = wave_sum(v.re);

@@ -77,6 +77,7 @@ struct Group {
   int wg_qr = 1, wg_inv = 1, wg_local = 1;
   int greens_route = 0;  // 0: D-scale separation, 1: verbatim reference route
   int local_variant = 0;  // 0: LDS-resident G when it fits, 1: G in global memory, 2: LDS-resident G
+  int udt_route = 0;      // 0: blocked QR with pre-pivoting, 1: column-pivoted QR (zgeqp3 semantics)
   bool recomputed = false;
   // kernel-class timing (dqmc_profile)
   unsigned prof_mask = 0;
@@ -206,6 +207,7 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
   ALLOC(d.qr_perm, int, n);
   ALLOC(d.qr_lpos, int, n);
   ALLOC(d.qr_tau, cplx, n);
+  ALLOC(d.qr_Tf, cplx, (size_t)16 * n);
   ALLOC(d.inv_piv, int, n);
   ALLOC(d.inv_bcast, cplx, 2 * n);
   ALLOC(d.lu_small, cplx, 32);
@@ -362,10 +364,16 @@ struct Ops {
     LCHK(launch_gemm(s(), d(), nc(), g.n, C, A, oa, B, ob, dl, dr, 0.0));
     return 0;
   }
-  int udt(MatRef A, VecRef D, MatRef Q, MatRef T) {
+  // decompose_udt! (linalg.jl:20-39). stack = true marks the column-graded products of the
+  // stack ((B...B U) diag(D), add_slice_sequence_*), which take the blocked pre-pivoted route
+  // unless udt route 1 is set; every other caller gets column-pivoted QR.
+  int udt(MatRef A, VecRef D, MatRef Q, MatRef T, bool stack = false) {
     {
       ProfScope ps(g, DQMC_K_QRP);
-      LCHK(launch_qrp(s(), d(), nc(), g.n, A, D, g.wg_qr));
+      if (stack && g.udt_route == 0)
+        LCHK(launch_qr_blocked(s(), d(), nc(), g.n, A, D, 1, g.wg_qr));
+      else
+        LCHK(launch_qrp(s(), d(), nc(), g.n, A, D, g.wg_qr));
     }
     ProfScope ps(g, DQMC_K_QFORM);
     LCHK(launch_form_TQ(s(), d(), nc(), g.n, A, D, Q, T));
@@ -448,7 +456,7 @@ struct Ops {
     for (int sl = idx * g.s; sl < (idx + 1) * g.s; ++sl)
       if ((rc = applyB(NM(M_CURU), B_LEFT, sl))) return rc;
     LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_CURU), DS(idx)));
-    if ((rc = udt(NM(M_CURU), DS(idx + 1), US(idx + 1), NM(M_QRT)))) return rc;
+    if ((rc = udt(NM(M_CURU), DS(idx + 1), US(idx + 1), NM(M_QRT), true))) return rc;
     return gemm(TS(idx + 1), NM(M_QRT), OP_N, TS(idx), OP_N);
   }
 
@@ -459,7 +467,7 @@ struct Ops {
     for (int sl = (idx + 1) * g.s - 1; sl >= idx * g.s; --sl)
       if ((rc = applyB(NM(M_CURU), B_DAG_LEFT, sl))) return rc;
     LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_CURU), DS(idx + 1)));
-    if ((rc = udt(NM(M_CURU), DS(idx), US(idx), NM(M_QRT)))) return rc;
+    if ((rc = udt(NM(M_CURU), DS(idx), US(idx), NM(M_QRT), true))) return rc;
     return gemm(TS(idx), NM(M_QRT), OP_N, TS(idx + 1), OP_N);
   }
 
@@ -484,9 +492,9 @@ struct Ops {
   // G = [1 + Ul Dl Tl (Ur Dr Tr)^H]^-1 with the D scales separated into Db = max(D, 1)
   // and Ds = min(D, 1):
   //   G = Ur Drb^-1 [Dlb^-1 Ul^H Ur Drb^-1 + Dls Tl Tr^H Drs]^-1 Dlb^-1 Ul^H,
-  // the bracket being well conditioned, so one partial-pivoting inverse replaces the two
-  // inner UDT decompositions of the verbatim route; log|det G| = -log|det [.]| - sum log Dlb
-  // - sum log Drb (Ul, Ur unitary). Always leaves log|det G| in S_LOGDET.
+  // the bracket being well conditioned, so one unpivoted Householder QR and a solve replace
+  // the two inner UDT decompositions of the verbatim route; log|det G| = -log|det [.]|
+  // - sum log Dlb - sum log Drb (Ul, Ur unitary). Always leaves log|det G| in S_LOGDET.
   int calculate_greens_scalesep() {
     int rc;
     LCHK(launch_split_scales(s(), d(), nc(), g.n, NV(V_DL), NV(V_D1), NV(V_D2)));  // 1/Dlb, Dls
@@ -496,13 +504,18 @@ struct Ops {
       LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_TMP), NM(M_UL), OP_C, NM(M_UR), OP_N, NV(V_D1), NV(V_D3), 0.0));
       LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_TMP), NM(M_TL), OP_N, NM(M_TR), OP_C, NV(V_D2), NV(V_D4), 1.0));
     }
-    if ((rc = inverse(NM(M_TMP)))) return rc;  // log|det [.]| -> S_LOGDET_A
-    LCHK(launch_scale_cols(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D1)));
     {
-      ProfScope ps(g, DQMC_K_GEMM);
-      LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_TMP), OP_N, NM(M_UL), OP_C, NV(V_D3), NOVEC, 0.0));
-      LCHK(launch_gemm(s(), d(), nc(), g.n, NM(M_G), NM(M_UR), OP_N, NM(M_TMP2), OP_N, NOVEC, NOVEC, 0.0));
+      // [.] = Q R (Householder, no pivoting: the matrix is well conditioned), then
+      // Y = [.]^-1 Dlb^-1 Ul^H = R^-1 Q^H (Dlb^-1 Ul^H); log|det [.]| = sum log |R_kk|
+      ProfScope ps(g, DQMC_K_INVERSE);
+      LCHK(launch_qr_blocked(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D5), 0, g.wg_qr));
+      LCHK(launch_adjoint(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_UL)));
+      LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_TMP2), NV(V_D1)));
+      LCHK(launch_qr_solve(s(), d(), nc(), g.n, NM(M_TMP), NM(M_TMP2)));
+      LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_TMP2), NV(V_D3)));
+      LCHK(launch_sum_log(s(), d(), nc(), g.n, NV(V_D5), S_LOGDET_A, 0));
     }
+    if ((rc = gemm(NM(M_G), NM(M_UR), OP_N, NM(M_TMP2), OP_N))) return rc;
     LCHK(launch_sum_log(s(), d(), nc(), g.n, NV(V_D1), S_LOGDET, 0));
     LCHK(launch_sum_log(s(), d(), nc(), g.n, NV(V_D3), S_LOGDET, 1));
     LCHK(launch_negate_scalar(s(), d(), nc(), S_LOGDET_A));
@@ -1083,6 +1096,13 @@ int dqmc_set_greens_route(dqmc_ctx* ctx, int route) {
   return 0;
 }
 
+int dqmc_set_udt_route(dqmc_ctx* ctx, int route) {
+  CTX_CHECK(ctx);
+  if (route != 0 && route != 1) return fail(DQMC_E_ARG, "udt route must be 0 or 1");
+  for (auto& g : ctx->b->groups) g.udt_route = route;
+  return 0;
+}
+
 int dqmc_set_local_variant(dqmc_ctx* ctx, int variant) {
   CTX_CHECK(ctx);
   if (variant < 0 || variant > 2) return fail(DQMC_E_ARG, "local variant must be 0, 1 or 2");
@@ -1247,6 +1267,24 @@ int dqmc_udt(dqmc_ctx* ctx, const double* A, double* U, double* D, double* T) {
   Ops o(ctx->b->groups[0]);
   RC(put_mat(ctx->b, NM(M_QRA), A));
   RC(o.udt(NM(M_QRA), NV(V_QRD), NM(M_QRQ), NM(M_QRT)));
+  RC(batch_sync(ctx->b));
+  if (U) RC(get_mat(ctx->b, NM(M_QRQ), U));
+  if (T) RC(get_mat(ctx->b, NM(M_QRT), T));
+  if (D) RC(get_vec(ctx->b, NV(V_QRD), D));
+  return 0;
+}
+
+int dqmc_udt_prepivoted(dqmc_ctx* ctx, const double* A, double* U, double* D, double* T) {
+  CTX_CHECK(ctx);
+  if (!A) return fail(DQMC_E_ARG, "null A");
+  Group& g = ctx->b->groups[0];
+  const int saved = g.udt_route;
+  g.udt_route = 0;
+  Ops o(g);
+  RC(put_mat(ctx->b, NM(M_QRA), A));
+  const int rc = o.udt(NM(M_QRA), NV(V_QRD), NM(M_QRQ), NM(M_QRT), true);
+  g.udt_route = saved;
+  RC(rc);
   RC(batch_sync(ctx->b));
   if (U) RC(get_mat(ctx->b, NM(M_QRQ), U));
   if (T) RC(get_mat(ctx->b, NM(M_QRT), T));

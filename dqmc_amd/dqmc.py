@@ -199,6 +199,10 @@ class DQMC:
     def set_groups(self, wg_qr=0, wg_inv=0, wg_local=0):
         check(self.lib.dqmc_ctx_set_groups(self.h, wg_qr, wg_inv, wg_local))
 
+    def set_udt_route(self, route):
+        """0: blocked QR with pre-pivoting for the stack (default); 1: column-pivoted QR."""
+        check(self.lib.dqmc_set_udt_route(self.h, route))
+
     def set_local_variant(self, variant):
         """0: G tiles in LDS when they fit (default); 1: G in global memory; 2: LDS only."""
         check(self.lib.dqmc_set_local_variant(self.h, variant))
@@ -298,6 +302,16 @@ class DQMC:
         T = np.zeros((n, n), dtype=np.complex128, order="F")
         D = np.zeros(n)
         check(self.lib.dqmc_udt(self.h, _cmat_ptr(A), _cmat_ptr(U), dptr(D), _cmat_ptr(T)))
+        return U, D, T
+
+    def decompose_udt_prepivoted(self, A):
+        """A = U diag(D) T by the stack's blocked QR with pre-pivoting (udt route 0)."""
+        n = self.p.n
+        A = _as_fcomplex(A, n)
+        U = np.zeros((n, n), dtype=np.complex128, order="F")
+        T = np.zeros((n, n), dtype=np.complex128, order="F")
+        D = np.zeros(n)
+        check(self.lib.dqmc_udt_prepivoted(self.h, _cmat_ptr(A), _cmat_ptr(U), dptr(D), _cmat_ptr(T)))
         return U, D, T
 
     def calculate_greens(self, Ul, Dl, Tl, Ur, Dr, Tr):

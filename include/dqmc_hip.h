@@ -108,6 +108,14 @@ int dqmc_set_greens_route(dqmc_ctx* ctx, int route);
  * memory; 1 always global memory; 2 always LDS (error when it does not fit). Same
  * results either way. */
 int dqmc_set_local_variant(dqmc_ctx* ctx, int variant);
+/* how the stack's decompose_udt! (linalg.jl:20-39) factors inside build_stack /
+ * add_slice_sequence_*: 0 (default) blocked Householder QR after pre-pivoting the
+ * columns by decreasing norm (two group barriers per 16-column panel), 1 column-
+ * pivoted QR with zgeqp3's pivot sequence (one barrier per column). Both are stable for
+ * the stack's column-graded products and give the same G; only the stack's internal
+ * U, D, T differ. Every other factorisation (dqmc_udt, the direct and time-displaced
+ * Green's functions, whose chains are also row-graded) uses route 1. */
+int dqmc_set_udt_route(dqmc_ctx* ctx, int route);
 
 /* p.hsfield (parameters.jl:19): phi[a + 3*(i + N*l)] */
 int dqmc_set_field(dqmc_ctx* ctx, const double* phi);
@@ -154,6 +162,9 @@ int dqmc_apply_B(dqmc_ctx* ctx, int slice, int which, double* X);
 int dqmc_wrap_greens(dqmc_ctx* ctx, double* G, int slice, int direction);
 /* decompose_udt (linalg.jl:10-39) of an n x n matrix */
 int dqmc_udt(dqmc_ctx* ctx, const double* A, double* U, double* D, double* T);
+/* No reference counterpart: the stack's default factorisation (udt route 0, blocked QR
+ * after pre-pivoting by column norm) as A = U diag(D) T, for tests */
+int dqmc_udt_prepivoted(dqmc_ctx* ctx, const double* A, double* U, double* D, double* T);
 /* calculate_greens + calculate_logdet (stack.jl:338-385) from given UDT factors */
 int dqmc_calculate_greens(dqmc_ctx* ctx, const double* Ul, const double* Dl, const double* Tl,
                           const double* Ur, const double* Dr, const double* Tr, double* G,

@@ -100,6 +100,32 @@ def test_udt_matches_lapack(amd):
     c.close()
 
 
+def test_udt_prepivoted_blocked_qr(amd):
+    """The stack's factorisation: U unitary, U diag(D) T = A column by column, T upper
+    triangular after the pre-pivoting order, D descending for a column-graded matrix."""
+    mod, DQMC = amd
+    rng = np.random.default_rng(7)
+    for L in (4, 10, 16):
+        _say("blocked pre-pivoted QR n=%d" % (4 * L * L))
+        c = DQMC(config_params(mod, L, 10))
+        n = c.p.n
+        scales = 10.0 ** rng.uniform(-30, 30, n)
+        A = rand_c(rng, n) * scales[None, :]
+        U, D, T = c.decompose_udt_prepivoted(A)
+        assert np.abs(U.conj().T @ U - np.eye(n)).max() < 1e-12
+        R = (U * D[None, :]) @ T
+        err = np.linalg.norm(R - A, axis=0) / np.linalg.norm(A, axis=0)
+        assert err.max() < 1e-12, err.max()
+        order = np.argsort(-np.linalg.norm(A, axis=0), kind="stable")
+        Tp = T[:, order]
+        assert np.abs(np.tril(Tp, -1)).max() == 0.0
+        assert np.allclose(np.abs(np.diag(Tp)), 1.0, atol=1e-12)
+        # the diagonal of R follows the grading of the sorted columns
+        d_ref = np.abs(np.diag(np.linalg.qr(A[:, order])[1]))
+        assert np.max(np.abs(np.log(D / d_ref))) < 1e-8
+        c.close()
+
+
 def test_multiply_safely_kat(amd):
     # tests_linalg.jl:105-113 through the device UDT (32x32 fixture embedded in n=64 is not
     # equivalent, so the product is formed at n = 32 ... padded with an identity block)
