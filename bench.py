@@ -7,7 +7,12 @@ phi in [-1,1]^3. One step = one sweep = M update steps (propagate + local
 updates of every site), the reference's printed "sweep duration"
 (dqmc_framework.jl:259-262). Prints one JSON line.
 
+The JSON line also carries the two other legs of the metric: "batch" (aggregate sweeps/s
+of 64 independent chains of the same workload advanced together) and "grad" (dS/dphi
+evaluations per second at L=12, beta=10).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch C]
+                       [--batch-sweeps S] [--grad-evals E]
 """
 import argparse
 import json
@@ -68,6 +73,49 @@ def cpu_baseline(p_amd, phi, slices=100):
                                                                                    slices // p.safe_mult)}
 
 
+def batch_measure(amd, nchains, sweeps, warmup):
+    """Aggregate sweeps/s of `nchains` independent chains at the same workload (L=10, beta=20),
+    advanced together by one batch (every kernel launch covers all chains)."""
+    from dqmc_amd import Batch
+    params = [workload_params(amd, seed=2000 + c) for c in range(nchains)]
+    b = Batch(params)
+    confs = [np.random.default_rng(p.seed).uniform(-1.0, 1.0, (3, p.N, p.slices)) for p in params]
+    b.init(confs)
+    if warmup > 0:
+        b.sweep(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.sweep(sweeps, sync=False)
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    sts = [b.stats(c) for c in range(nchains)]
+    b.close()
+    return {"chains": nchains, "value": nchains * sweeps / dt, "unit": "sweeps/s (all chains)",
+            "per_chain": sweeps / dt, "sweeps": sweeps, "warmup": warmup, "seconds": dt,
+            "drift_max": max(st.drift_max for st in sts),
+            "acceptance": sum(st.accepted for st in sts) / max(sum(st.proposed for st in sts), 1)}
+
+
+def grad_measure(amd, evals, warmup):
+    """dS/dphi evaluations per second at L=12, beta=10 (n=576, M=100): one stabilised pass over
+    all slices with the trace formula at every (component, site, slice)."""
+    from dqmc_amd import DQMC, UniformStream
+    p = workload_params(amd, L=12, slices=100, seed=3001)
+    phi = np.random.default_rng(p.seed).uniform(-1.0, 1.0, (3, p.N, p.slices))
+    c = DQMC(p, uniforms=UniformStream(p.seed))
+    c.init(phi)
+    for _ in range(warmup):
+        c.grad()
+    c.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        g, _s = c.grad()
+    dt = time.perf_counter() - t0
+    c.close()
+    return {"value": evals / dt, "unit": "evals/s", "config": "L=12 beta=10 dtau=0.1 (M=100), n=576",
+            "components": int(g.size), "evals": evals, "seconds": dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +123,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=100)
+    ap.add_argument("--batch", type=int, default=64, help="chains of the batch leg (0 skips it)")
+    ap.add_argument("--batch-sweeps", type=int, default=1)
+    ap.add_argument("--grad-evals", type=int, default=3, help="0 skips the gradient leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: a DQMC chain does not shard across GPUs and independent "
@@ -148,6 +199,10 @@ def main():
         "kernel_ms_per_sweep_warmup": warm_breakdown,
     }
     chain.close()
+    if args.batch > 0:
+        out["batch"] = batch_measure(dqmc_amd, args.batch, args.batch_sweeps, 1)
+    if args.grad_evals > 0:
+        out["grad"] = grad_measure(dqmc_amd, args.grad_evals, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, phi, args.cpu_slices)
     print(json.dumps(out))
