@@ -83,6 +83,9 @@ struct ChainDesc {
   cplx* lu_small;         // [32] delta (16) and M^-1 (16) of the last single proposal
   cplx* lu_win;           // [2][4 sites][4][n] columns published between local-update rounds
   cplx* lu_diag;          // [2][N][16] diagonal site blocks published between rounds
+  cplx* ld_A;             // [16][n][4] pending left factors of the delayed local updates
+  cplx* ld_B;             // [16][4][n] pending right factors
+  int* ld_cmd;            // [2] command and pending count of a flush round
   // local updates
   const double* uniforms; // host-drawn uniform stream (device copy)
   long long* ucursor;     // next unread uniform
@@ -206,11 +209,14 @@ hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, 
 // X <- X^-1 in place (Gauss-Jordan, partial pivoting); scal[S_LOGDET_A] = log|det X|.
 hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);
 
-// One local-update pass over all sites at `slice` (0-based). tile = 1 keeps each workgroup's
-// column block of G in LDS (needs local_tile_fits), 0 updates G in global memory.
+// One local-update pass over all sites at `slice` (0-based). variant 1 updates G in global
+// memory after every accepted proposal, 2 keeps each workgroup's column block of G in LDS
+// (needs local_tile_fits), 3 delays the rank-4 updates and flushes them in groups (needs
+// local_delayed_fits).
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
-                                const int* slices, double box, int wg, int log, int tile);
+                                const int* slices, double box, int wg, int log, int variant);
 bool local_tile_fits(int nmax, int wg);
+bool local_delayed_fits(int nmax);
 
 // scal[S_LOGDET] = log|det UG| + sum log DG + log|det TG|  helper pieces
 hipError_t launch_sum_log(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef d, int slot,

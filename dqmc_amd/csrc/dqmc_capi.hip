@@ -76,7 +76,7 @@ struct Group {
   int cur = 0, dir = 1;  // reference current_slice (1-based), direction
   int wg_qr = 1, wg_inv = 1, wg_local = 1;
   int greens_route = 0;  // 0: D-scale separation, 1: verbatim reference route
-  int local_variant = 0;  // 0: LDS-resident G when it fits, 1: G in global memory, 2: LDS-resident G
+  int local_variant = 0;  // 0 auto, 1 G in global memory, 2 LDS-resident G tiles, 3 delayed updates
   int udt_route = 0;      // 0: blocked QR with pre-pivoting, 1: column-pivoted QR (zgeqp3 semantics)
   bool recomputed = false;
   // kernel-class timing (dqmc_profile)
@@ -213,6 +213,9 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
   ALLOC(d.lu_small, cplx, 32);
   ALLOC(d.lu_win, cplx, (size_t)2 * 4 * 4 * n);
   ALLOC(d.lu_diag, cplx, (size_t)2 * 16 * N);
+  ALLOC(d.ld_A, cplx, (size_t)16 * 4 * n);
+  ALLOC(d.ld_B, cplx, (size_t)16 * 4 * n);
+  ALLOC(d.ld_cmd, int, 2);
   ALLOC(d.ucursor, long long, 1);
   ALLOC(d.scal, double, kNumScalar);
   ALLOC(d.log_count, long long, 1);
@@ -835,10 +838,13 @@ struct Ops {
   int local_updates(double box, int dolog) {
     if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "local_updates outside slices 1..M");
     ProfScope ps(g, DQMC_K_LOCAL);
-    const bool fits = local_tile_fits(g.n, g.wg_local);
-    if (g.local_variant == 2 && !fits) return fail(DQMC_E_ARG, "G tiles do not fit in LDS at this workgroup count");
-    const int tile = g.local_variant == 2 || (g.local_variant == 0 && fits);
-    LCHK(launch_local_updates(s(), d(), nc(), g.n, g.cur - 1, nullptr, box, g.wg_local, dolog, tile));
+    const bool tile_fits = local_tile_fits(g.n, g.wg_local), delay_fits = local_delayed_fits(g.n);
+    if (g.local_variant == 2 && !tile_fits)
+      return fail(DQMC_E_ARG, "G tiles do not fit in LDS at this workgroup count");
+    if (g.local_variant == 3 && !delay_fits) return fail(DQMC_E_ARG, "delayed updates do not fit in LDS at this size");
+    int variant = g.local_variant;
+    if (variant == 0) variant = tile_fits ? 2 : delay_fits ? 3 : 1;
+    LCHK(launch_local_updates(s(), d(), nc(), g.n, g.cur - 1, nullptr, box, g.wg_local, dolog, variant));
     return 0;
   }
 };
@@ -1105,7 +1111,7 @@ int dqmc_set_udt_route(dqmc_ctx* ctx, int route) {
 
 int dqmc_set_local_variant(dqmc_ctx* ctx, int variant) {
   CTX_CHECK(ctx);
-  if (variant < 0 || variant > 2) return fail(DQMC_E_ARG, "local variant must be 0, 1 or 2");
+  if (variant < 0 || variant > 3) return fail(DQMC_E_ARG, "local variant must be 0, 1, 2 or 3");
   for (auto& g : ctx->b->groups) g.local_variant = variant;
   return 0;
 }

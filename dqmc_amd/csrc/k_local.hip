@@ -597,6 +597,231 @@ __global__ __launch_bounds__(256) void k_local_updates_tile(const ChainDesc* cha
   if (w == 0) finish_slice(c, l, s, p, dolog, stop);
 }
 
+// ---------------------------------------------------------------------------
+// Variant 3: delayed updates. Workgroup 0 runs the proposals alone and keeps every
+// accepted rank-4 update as a pair of factors, A_a = (G'[:, c] - E) M^-1 (n x 4) and
+// B_a = delta G'[r, :] (4 x n), where G' = G + sum_{pending} A B is the current Green's
+// function; the diagonal site blocks of the sites still to visit are kept current in LDS
+// (one 4x4 correction per accepted update and remaining site), so decisions read only LDS.
+// After kLocalDelay pending updates (and at the end of the slice) every workgroup of the
+// chain adds sum_a A_a B_a to its own columns of G: two group barriers per flush instead
+// of one per accepted proposal, and G is streamed once per flush.
+// Same arithmetic as update_greens! (local_updates.jl:61-95) up to the order of sums.
+// ---------------------------------------------------------------------------
+constexpr int kLocalDelay = 8;
+enum : int { LD_CONTINUE = 0, LD_FLUSH = 1, LD_DONE = 2, LD_FLUSH_DONE = 3 };
+
+__host__ __device__ inline size_t delayed_lds_bytes(int n, int N) {
+  return slice_lds_doubles(N) * 8 + (size_t)N * 16 * 16 + (size_t)8 * n * 16;
+}
+
+// G[:, cb0:cb1] += sum_{a < m} A_a B_a[:, cb0:cb1], B staged through LDS in column chunks
+__device__ inline void delayed_flush(const ChainDesc& c, int m, int cb0, int cb1, cplx* stage, int stage_cols) {
+  const int n = c.n, tid = threadIdx.x;
+  cplx* G = c.mat[M_G];
+  const cplx* As = c.ld_A;
+  const cplx* Bs = c.ld_B;
+  const int kq = 4 * m;
+  for (int j0 = cb0; j0 < cb1; j0 += stage_cols) {
+    const int nc = min(stage_cols, cb1 - j0);
+    __syncthreads();
+    for (int e = tid; e < kq * nc; e += blockDim.x) {
+      const int aq = e / nc, jl = e - aq * nc;
+      stage[e] = Bs[(size_t)aq * n + j0 + jl];  // B_a[q][col], aq = 4 a + q
+    }
+    __syncthreads();
+    for (int r = tid; r < n; r += blockDim.x) {
+      for (int a0 = 0; a0 < m; a0 += 4) {
+        cplx ar[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int a = a0 + (t >> 2);
+          ar[t] = a < m ? As[((size_t)a * n + r) * 4 + (t & 3)] : mk(0, 0);
+        }
+        for (int jl = 0; jl < nc; ++jl) {
+          cplx acc = mk(0, 0);
+#pragma unroll
+          for (int t = 0; t < 16; ++t)
+            if (a0 + (t >> 2) < m) cfma(acc, ar[t], stage[(size_t)(4 * a0 + t) * nc + jl]);
+          cplx* gp = G + (size_t)(j0 + jl) * n + r;
+          *gp = *gp + acc;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_local_updates_delayed(const ChainDesc* chains, int slice, const int* slices,
+                                                               double box, int dolog) {
+  const ChainDesc& c = chains[blockIdx.y];
+  const int n = c.n, N = c.N;
+  const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+  const int l = slices ? slices[blockIdx.y] : slice;
+  GroupSync g = chain_group(c);
+  const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
+
+  extern __shared__ double smem[];
+  cplx* Gd = reinterpret_cast<cplx*>(smem + slice_lds_doubles(N));  // [N][16] current diagonal blocks
+  cplx* Am = Gd + (size_t)N * 16;                                   // [n][4] A of the last accepted
+  cplx* Bm = Am + (size_t)4 * n;                                    // [4][n] B of the last accepted
+  cplx* stage = Am;                                                  // flush staging (8n entries)
+  const int stage_cols = (8 * n) / (4 * kLocalDelay);
+  __shared__ LocalShared sh;
+  __shared__ cplx bcol[kLocalDelay * 16];  // B_a[:, c_i] of the pending updates
+  __shared__ cplx arow[kLocalDelay * 16];  // A_a[r_i, :] of the pending updates
+  __shared__ int s_cmd;
+  cplx* G = c.mat[M_G];
+
+  if (w != 0) {
+    // flush helper: wait for workgroup 0's command at each round
+    for (;;) {
+      group_sync(g);
+      if (group_failed(g)) return;
+      const int cmd = *reinterpret_cast<volatile int*>(c.ld_cmd);
+      const int m = *reinterpret_cast<volatile int*>(c.ld_cmd + 1);
+      if (cmd == LD_FLUSH || cmd == LD_FLUSH_DONE) {
+        delayed_flush(c, m, cb0, cb1, stage, stage_cols);
+        group_sync(g);
+        if (group_failed(g)) return;
+      }
+      if (cmd == LD_DONE || cmd == LD_FLUSH_DONE) return;
+    }
+  }
+
+  const SliceLds s = load_slice(c, l, smem);
+  for (int e = tid; e < N * 16; e += blockDim.x) {
+    const int j = e >> 4, a = (e >> 2) & 3, b = e & 3;
+    Gd[e] = G[(size_t)(j + b * N) * n + (j + a * N)];
+  }
+  __syncthreads();
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  int m = 0, stop = 0;
+  cplx* As = c.ld_A;
+  cplx* Bs = c.ld_B;
+
+  auto round = [&](int cmd) -> bool {
+    if (tid == 0) {
+      c.ld_cmd[0] = cmd;
+      c.ld_cmd[1] = m;
+    }
+    group_sync(g);
+    if (group_failed(g)) return false;
+    if (cmd == LD_FLUSH || cmd == LD_FLUSH_DONE) {
+      delayed_flush(c, m, cb0, cb1, stage, stage_cols);
+      group_sync(g);
+      if (group_failed(g)) return false;
+    }
+    return true;
+  };
+
+  for (;;) {
+    if (tid < 64) {
+      const int lane = tid & 63, ea = (lane & 15) >> 2, eb = lane & 3;
+      int st;
+      const int site = evaluate_proposals(c, s, p, sh, box, dolog, true,
+                                          [&](int j) { return Gd[16 * j + 4 * ea + eb]; }, &st);
+      if (tid == 0) {
+        sh.site = site;
+        sh.stop = st;
+      }
+    }
+    __syncthreads();
+    const int site = sh.site;
+    stop = sh.stop;
+    if (site >= N) break;
+    // corrections of the pending updates at the site's columns and rows
+    for (int e = tid; e < m * 16; e += blockDim.x) {
+      const int a = e >> 4, q = (e >> 2) & 3, k = e & 3;
+      bcol[e] = Bs[((size_t)a * 4 + q) * n + site + k * N];        // B_a[q][c_k]
+      arow[e] = As[((size_t)a * n + site + q * N) * 4 + k];        // A_a[r_q][k]
+    }
+    __syncthreads();
+    // A = (G'[:, c] - E) M^-1, thread per row; every load of a row is issued before use
+    for (int r = tid; r < n; r += blockDim.x) {
+      cplx g4[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) g4[k] = G[(size_t)(site + k * N) * n + r];
+      for (int a0 = 0; a0 < m; a0 += 4) {
+        cplx av[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+          av[t] = a0 + (t >> 2) < m ? As[((size_t)(a0 + (t >> 2)) * n + r) * 4 + (t & 3)] : mk(0, 0);
+#pragma unroll
+        for (int da = 0; da < 4; ++da) {
+          const int a = a0 + da;
+          if (a >= m) break;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const cplx* bc = bcol + a * 16 + k;
+            cfma(g4[k], av[4 * da + 0], bc[0]);
+            cfma(g4[k], av[4 * da + 1], bc[4]);
+            cfma(g4[k], av[4 * da + 2], bc[8]);
+            cfma(g4[k], av[4 * da + 3], bc[12]);
+          }
+        }
+      }
+      for (int k = 0; k < 4; ++k)
+        if (r == site + k * N) g4[k].re -= 1.0;
+      for (int k = 0; k < 4; ++k) {
+        cplx acc = mk(0, 0);
+        for (int q = 0; q < 4; ++q) cfma(acc, g4[q], sh.minv[q * 4 + k]);
+        Am[4 * (size_t)r + k] = acc;
+        As[((size_t)m * n + r) * 4 + k] = acc;
+      }
+    }
+    // B = delta G'[r, :], thread per column; loads issued together as above
+    for (int col = tid; col < n; col += blockDim.x) {
+      cplx g4[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g4[q] = G[(size_t)col * n + site + q * N];
+      for (int a0 = 0; a0 < m; a0 += 4) {
+        cplx bv[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) bv[t] = a0 + (t >> 2) < m ? Bs[(size_t)(4 * a0 + t) * n + col] : mk(0, 0);
+#pragma unroll
+        for (int da = 0; da < 4; ++da) {
+          const int a = a0 + da;
+          if (a >= m) break;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const cplx* ar = arow + a * 16 + q * 4;
+            cfma(g4[q], ar[0], bv[4 * da + 0]);
+            cfma(g4[q], ar[1], bv[4 * da + 1]);
+            cfma(g4[q], ar[2], bv[4 * da + 2]);
+            cfma(g4[q], ar[3], bv[4 * da + 3]);
+          }
+        }
+      }
+      for (int k = 0; k < 4; ++k) {
+        cplx acc = mk(0, 0);
+        for (int q = 0; q < 4; ++q) cfma(acc, sh.delta[k * 4 + q], g4[q]);
+        Bm[(size_t)k * n + col] = acc;
+        Bs[((size_t)m * 4 + k) * n + col] = acc;
+      }
+    }
+    __syncthreads();
+    // diagonal blocks of the sites still to visit: Gd_j += A[r_j, :] B[:, c_j]
+    for (int e = tid; e < (N - site - 1) * 16; e += blockDim.x) {
+      const int j = site + 1 + (e >> 4), a = (e >> 2) & 3, b = e & 3;
+      const cplx* ar = Am + 4 * (size_t)(j + a * N);
+      cplx acc = Gd[16 * j + 4 * a + b];
+      for (int q = 0; q < 4; ++q) cfma(acc, ar[q], Bm[(size_t)q * n + j + b * N]);
+      Gd[16 * j + 4 * a + b] = acc;
+    }
+    ++m;
+    __syncthreads();
+    if (m == kLocalDelay) {
+      if (!round(LD_FLUSH)) return;
+      m = 0;
+    }
+  }
+  if (!round(m > 0 ? LD_FLUSH_DONE : LD_DONE)) return;
+  finish_slice(c, l, s, p, dolog, stop);
+}
+
+bool local_delayed_fits(int nmax) { return delayed_lds_bytes(nmax, nmax / 4) <= 150 * 1024; }
+
 // LDS bytes of the two variants at the given shape (dynamic part)
 static size_t local_lds_global(int n, int N, int wg) {
   return slice_lds_doubles(N) * 8 + (size_t)4 * n * 16 + (size_t)4 * ((n + wg - 1) / wg + 1) * 16;
@@ -610,16 +835,20 @@ constexpr size_t kLocalLdsMax = 150 * 1024;
 bool local_tile_fits(int nmax, int wg) { return local_lds_tile(nmax, nmax / 4, wg) <= kLocalLdsMax; }
 
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
-                                const int* slices, double box, int wg, int log, int tile) {
+                                const int* slices, double box, int wg, int log, int variant) {
   const int Nmax = nmax / 4;
-  const void* fn = tile ? (const void*)k_local_updates_tile : (const void*)k_local_updates;
-  const size_t lds = tile ? local_lds_tile(nmax, Nmax, wg) : local_lds_global(nmax, Nmax, wg);
+  const void* fn = variant == 2   ? (const void*)k_local_updates_tile
+                   : variant == 3 ? (const void*)k_local_updates_delayed
+                                  : (const void*)k_local_updates;
+  const size_t lds = variant == 2   ? local_lds_tile(nmax, Nmax, wg)
+                     : variant == 3 ? delayed_lds_bytes(nmax, Nmax)
+                                    : local_lds_global(nmax, Nmax, wg);
   if (lds > kLocalLdsMax) return hipErrorInvalidValue;
-  static size_t attr[2] = {0, 0};
-  if (lds > 65536 && lds > attr[tile ? 1 : 0]) {
+  static size_t attr[4] = {0, 0, 0, 0};
+  if (lds > 65536 && lds > attr[variant & 3]) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    attr[tile ? 1 : 0] = lds;
+    attr[variant & 3] = lds;
   }
   void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log};
   hipError_t e = hipLaunchCooperativeKernel(fn, dim3(wg, nchains), dim3(256), args, lds, s);

@@ -25,6 +25,7 @@ constexpr int kQrMaxN = 1024;
 constexpr int kQrMaxNb = 16;
 
 __device__ inline cplx wsum(cplx v) { return wave_sum(v); }
+__device__ inline cplx shfl_c(cplx v, int src) { return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64)); }
 
 __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, MatRef Ar, VecRef Dr, int prepivot,
                                                     int nb) {
@@ -41,6 +42,7 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   __shared__ int perm[kQrMaxN], lpos[kQrMaxN];
   __shared__ cplx Tl[kQrMaxNb * kQrMaxNb];  // Tl[j * nb + t] = T(j, t)
   __shared__ cplx taul[kQrMaxNb];
+  __shared__ double betal[kQrMaxNb];
   __shared__ double red[8];
 
   // ---- column order
@@ -83,33 +85,38 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
         P[(size_t)t * n + r] = A[(size_t)perm[k0 + t] * n + r];
       }
       __syncthreads();
+      // per column: wave 0 forms the reflector (zlarfg) alone, one barrier publishes it, every
+      // wave applies it to its share of the remaining panel columns, one barrier
       for (int t = 0; t < kb; ++t) {
         const int i = k0 + t;
         cplx* x = P + (size_t)t * n;
-        double part = 0.0;
-        for (int k = i + 1 + tid; k < n; k += blockDim.x) part += abs2(x[k]);
-        const double xnorm = sqrt(block_sum(part, red));
-        const cplx alpha = x[i];
-        cplx tau;
-        double beta;
-        // zlarfg: H^H (alpha, x) = (beta, 0), v = (1, x / (alpha - beta))
-        if (xnorm == 0.0 && alpha.im == 0.0) {
-          tau = mk(0, 0);
-          beta = alpha.re;
-        } else {
-          beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm * xnorm), alpha.re);
-          tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
-          const cplx sc = cinv(alpha - mk(beta, 0));
-          for (int k = i + 1 + tid; k < n; k += blockDim.x) x[k] = x[k] * sc;
+        if (wave == 0) {
+          double part = 0.0;
+          for (int k = i + 1 + lane; k < n; k += 64) part += abs2(x[k]);
+          const double xnorm = sqrt(wave_sum(part));
+          const cplx alpha = x[i];
+          cplx tau;
+          double beta;
+          // zlarfg: H^H (alpha, x) = (beta, 0), v = (1, x / (alpha - beta))
+          if (xnorm == 0.0 && alpha.im == 0.0) {
+            tau = mk(0, 0);
+            beta = alpha.re;
+          } else {
+            beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm * xnorm), alpha.re);
+            tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+            const cplx sc = cinv(alpha - mk(beta, 0));
+            for (int k = i + 1 + lane; k < n; k += 64) x[k] = x[k] * sc;
+          }
+          if (lane == 0) {
+            taul[t] = tau;
+            betal[t] = beta;
+            D[i] = fabs(beta);
+            c.qr_tau[i] = tau;
+          }
         }
         __syncthreads();
-        if (tid == 0) {
-          taul[t] = tau;
-          D[i] = fabs(beta);
-          c.qr_tau[i] = tau;
-        }
         // H^H = I - conj(tau) v v^H on the remaining panel columns, one wave per column
-        const cplx ctau = conj(tau);
+        const cplx ctau = conj(taul[t]);
         for (int cc = t + 1 + wave; cc < kb; cc += nwaves) {
           cplx* y = P + (size_t)cc * n;
           cplx acc = mk(0, 0);
@@ -124,9 +131,9 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
           }
         }
         __syncthreads();
-        if (tid == 0) x[i] = mk(beta, 0);
-        __syncthreads();
       }
+      // R's diagonal goes where the reflectors' implicit ones were (never read again)
+      if (tid < kb) P[(size_t)tid * n + k0 + tid] = mk(betal[tid], 0);
       // ---- zlarft (forward, columnwise): T(t,t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) V(:, 0:t)^H v_t
       // first every inner product V(:, j)^H v_t, j < t, into Tl(j, t)
       for (int q = wave; q < kb * kb; q += nwaves) {
@@ -280,16 +287,31 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
     }
   }
   __syncthreads();
-  // back substitution R y = z, one wave per column, column-oriented from the last row
-  for (int jl = wave; jl < ncol; jl += nwaves) {
-    cplx* y = bc + (size_t)jl * n;
-    for (int k = n - 1; k >= 0; --k) {
-      const cplx* rk = A + (size_t)k * n;
-      const cplx yk = y[k] * cinv(rk[k]);
-      __builtin_amdgcn_wave_barrier();
-      for (int r = lane; r < k; r += 64) y[r] = y[r] - rk[r] * yk;
-      if (lane == 0) y[k] = yk;
-      __builtin_amdgcn_wave_barrier();
+  // back substitution R y = z in blocks of 16 rows from the bottom: one wave per column solves
+  // the 16 x 16 diagonal block (lane r holds row lo + r), then all threads subtract the block's
+  // contribution from the rows above, reading R's block column once, coalesced
+  for (int hi = n; hi > 0; hi -= 16) {
+    const int lo = max(0, hi - 16), bs = hi - lo;
+    __syncthreads();
+    for (int jl = wave; jl < ncol; jl += nwaves) {
+      cplx* y = bc + (size_t)jl * n;
+      const int r = lo + (lane & 15);
+      cplx zr = (lane < bs) ? y[r] : mk(0, 0);
+      for (int k = bs - 1; k >= 0; --k) {
+        const cplx rkk = A[(size_t)(lo + k) * n + lo + k];
+        const cplx yk = shfl_c(zr, k) * cinv(rkk);
+        if (lane < k) zr = zr - A[(size_t)(lo + k) * n + r] * yk;
+        if (lane == k) zr = yk;
+      }
+      if (lane < bs) y[r] = zr;
+    }
+    __syncthreads();
+    for (int e = tid; e < lo * ncol; e += blockDim.x) {
+      const int jl = e / lo, r = e - jl * lo;
+      cplx* y = bc + (size_t)jl * n;
+      cplx acc = y[r];
+      for (int k = lo; k < hi; ++k) acc = acc - A[(size_t)k * n + r] * y[k];
+      y[r] = acc;
     }
   }
   __syncthreads();

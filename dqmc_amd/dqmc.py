@@ -204,7 +204,7 @@ class DQMC:
         check(self.lib.dqmc_set_udt_route(self.h, route))
 
     def set_local_variant(self, variant):
-        """0: G tiles in LDS when they fit (default); 1: G in global memory; 2: LDS only."""
+        """0 auto (default); 1 G in global memory; 2 LDS-resident G tiles; 3 delayed updates."""
         check(self.lib.dqmc_set_local_variant(self.h, variant))
 
     def set_greens_route(self, route):

@@ -103,10 +103,12 @@ int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local);
  * inner UDT decompositions, myrdiv and inv). Both give the same G and log|det G|
  * to rounding; the setting applies to every later recompute of the context. */
 int dqmc_set_greens_route(dqmc_ctx* ctx, int route);
-/* where local_updates (local_updates.jl:1-39) keeps G while it sweeps the sites of a
- * slice: 0 (default) each workgroup's column block in LDS when it fits, else global
- * memory; 1 always global memory; 2 always LDS (error when it does not fit). Same
- * results either way. */
+/* how local_updates (local_updates.jl:1-39) applies update_greens! while it sweeps the
+ * sites of a slice: 1 updates G in global memory after every accepted proposal; 2 keeps
+ * each workgroup's column block of G in LDS; 3 delays the rank-4 updates (up to 8 pending,
+ * decisions from LDS-resident diagonal blocks) and adds them to G in one pass; 0 (default)
+ * picks 2, else 3, else 1, by what fits in LDS. Same results to rounding; an error when a
+ * forced variant does not fit. */
 int dqmc_set_local_variant(dqmc_ctx* ctx, int variant);
 /* how the stack's decompose_udt! (linalg.jl:20-39) factors inside build_stack /
  * add_slice_sequence_*: 0 (default) blocked Householder QR after pre-pivoting the

@@ -349,10 +349,10 @@ def test_seeded_chain_small(amd):
     c2.close()
 
 
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 def test_local_update_variants(amd, variant):
-    """G in global memory (1) and G tiles resident in LDS (2): same decisions as the oracle,
-    G within 1e-10, for several workgroup counts (L=6, beta=2)."""
+    """G in global memory (1), G tiles resident in LDS (2) and delayed updates (3): same
+    decisions as the oracle, G within 1e-10, for several workgroup counts (L=6, beta=2)."""
     mod, DQMC = amd
     for wg in (3, 8, 36):
         _say("local-update variant %d, %d workgroups, L=6 M=20, 3 sweeps" % (variant, wg))
