@@ -51,6 +51,20 @@ def algorithmic_work(cls, n, acc_per_launch):
     return "hbm", 32.0 * n * n, "B"
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_L10_beta20_single_chain.json")
+
+
+def measured_traffic(kernel_class):
+    """HBM bytes per launch of a kernel class at this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size); None if absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            k = json.load(f)["kernels"].get(kernel_class)
+        return None if k is None else float(k["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def cpu_baseline(p_amd, phi, slices=100):
     """The oracle (numpy + OpenBLAS LAPACK, 1 BLAS thread) on `slices` update steps of the
     same workload after its init; scaled to sweeps/s."""
@@ -126,6 +140,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="chains of the batch leg (0 skips it)")
     ap.add_argument("--batch-sweeps", type=int, default=1)
     ap.add_argument("--grad-evals", type=int, default=3, help="0 skips the gradient leg")
+    ap.add_argument("--pmc-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a kernel of known traffic that "
+                         "calibrates FETCH_SIZE / WRITE_SIZE counter runs); 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: a DQMC chain does not shard across GPUs and independent "
@@ -135,6 +152,14 @@ def main():
     import dqmc_amd
     from dqmc_amd import DQMC, UniformStream
     from dqmc_amd import _capi
+
+    if args.pmc_copy_mib > 0:
+        import torch
+        src = torch.ones(args.pmc_copy_mib << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     p = workload_params(dqmc_amd)
     rng = np.random.default_rng(p.seed)
@@ -191,7 +216,9 @@ def main():
                                "Assaad checkerboard, safe_mult=10", "L": p.L, "n": p.n, "slices": p.slices,
                    "chains": 1},
         "roofline": {"bound": bound, "kernel": _capi.K_NAMES[dominant], "achieved": achieved, "peak": peak,
-                     "unit": runit, "frac": achieved / peak, "traffic": None,
+                     "unit": runit, "frac": achieved / peak,
+                     "traffic": measured_traffic(_capi.K_NAMES[dominant]),
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                      "work_per_launch": work, "work_unit": unit, "avg_launch_ms": avg_ms,
                      "launches": int(cnt[dominant])},
         "drift_max": st.drift_max,
