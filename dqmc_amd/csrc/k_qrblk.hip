@@ -293,14 +293,19 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
   for (int hi = n; hi > 0; hi -= 16) {
     const int lo = max(0, hi - 16), bs = hi - lo;
     __syncthreads();
+    // the 16 x 16 diagonal block of R and its reciprocal diagonal, staged once
+    for (int e = tid; e < bs * bs; e += blockDim.x) {
+      const int kk = e / bs, rr = e - kk * bs;
+      Tl[kk * 16 + rr] = A[(size_t)(lo + kk) * n + lo + rr];
+    }
+    __syncthreads();
     for (int jl = wave; jl < ncol; jl += nwaves) {
       cplx* y = bc + (size_t)jl * n;
       const int r = lo + (lane & 15);
       cplx zr = (lane < bs) ? y[r] : mk(0, 0);
       for (int k = bs - 1; k >= 0; --k) {
-        const cplx rkk = A[(size_t)(lo + k) * n + lo + k];
-        const cplx yk = shfl_c(zr, k) * cinv(rkk);
-        if (lane < k) zr = zr - A[(size_t)(lo + k) * n + r] * yk;
+        const cplx yk = shfl_c(zr, k) * cinv(Tl[k * 16 + k]);
+        if (lane < k) zr = zr - Tl[k * 16 + (lane & 15)] * yk;
         if (lane == k) zr = yk;
       }
       if (lane < bs) y[r] = zr;
