@@ -107,49 +107,69 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
             const cplx sc = cinv(alpha - mk(beta, 0));
             for (int k = i + 1 + lane; k < n; k += 64) x[k] = x[k] * sc;
           }
-          if (lane == 0) {
-            taul[t] = tau;
+          if (lane == 0) {  // global copies are written after the panel: a store here
+            taul[t] = tau;   // would make every later __syncthreads wait for it
             betal[t] = beta;
-            D[i] = fabs(beta);
-            c.qr_tau[i] = tau;
           }
         }
         __syncthreads();
-        // H^H = I - conj(tau) v v^H on the remaining panel columns, one wave per column
+        // H^H = I - conj(tau) v v^H on the remaining panel columns: wave w takes columns
+        // t+1+w, t+5+w, ... together, so their loads and sums overlap
         const cplx ctau = conj(taul[t]);
-        for (int cc = t + 1 + wave; cc < kb; cc += nwaves) {
-          cplx* y = P + (size_t)cc * n;
-          cplx acc = mk(0, 0);
+        {
+          constexpr int kC = kQrMaxNb / 4;
+          cplx acc[kC];
+#pragma unroll
+          for (int u = 0; u < kC; ++u) acc[u] = mk(0, 0);
           for (int k = i + lane; k < n; k += 64) {
             const cplx vk = (k == i) ? mk(1, 0) : x[k];
-            cfma_cj(acc, vk, y[k]);
+#pragma unroll
+            for (int u = 0; u < kC; ++u) {
+              const int cc = t + 1 + wave + 4 * u;
+              if (cc < kb) cfma_cj(acc[u], vk, P[(size_t)cc * n + k]);
+            }
           }
-          const cplx f = ctau * wsum(acc);
+          cplx f[kC];
+#pragma unroll
+          for (int u = 0; u < kC; ++u) f[u] = (t + 1 + wave + 4 * u < kb) ? ctau * wsum(acc[u]) : mk(0, 0);
           for (int k = i + lane; k < n; k += 64) {
             const cplx vk = (k == i) ? mk(1, 0) : x[k];
-            y[k] = y[k] - f * vk;
+#pragma unroll
+            for (int u = 0; u < kC; ++u) {
+              const int cc = t + 1 + wave + 4 * u;
+              if (cc < kb) P[(size_t)cc * n + k] = P[(size_t)cc * n + k] - f[u] * vk;
+            }
           }
         }
         __syncthreads();
       }
       // R's diagonal goes where the reflectors' implicit ones were (never read again)
-      if (tid < kb) P[(size_t)tid * n + k0 + tid] = mk(betal[tid], 0);
+      if (tid < kb) {
+        P[(size_t)tid * n + k0 + tid] = mk(betal[tid], 0);
+        D[k0 + tid] = fabs(betal[tid]);
+        c.qr_tau[k0 + tid] = taul[tid];
+      }
       // ---- zlarft (forward, columnwise): T(t,t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) V(:, 0:t)^H v_t
       // first every inner product V(:, j)^H v_t, j < t, into Tl(j, t)
-      for (int q = wave; q < kb * kb; q += nwaves) {
-        const int j = q / kb, t = q - (q / kb) * kb;
-        if (j >= t) continue;
-        const cplx* vj = P + (size_t)j * n;
+      // (wave w takes t = w, w+4, ...; the t inner products of one t share v_t's loads)
+      for (int t = 1 + wave; t < kb; t += nwaves) {
         const cplx* vt = P + (size_t)t * n;
         const int it = k0 + t;
-        cplx acc = mk(0, 0);
+        cplx acc[kQrMaxNb];
+#pragma unroll
+        for (int j = 0; j < kQrMaxNb; ++j) acc[j] = mk(0, 0);
         for (int k = it + lane; k < n; k += 64) {
-          const cplx a = vj[k];  // row it > k0 + j: a below-diagonal entry of v_j
           const cplx b = (k == it) ? mk(1, 0) : vt[k];
-          cfma_cj(acc, a, b);
+#pragma unroll
+          for (int j = 0; j < kQrMaxNb; ++j)
+            if (j < t) cfma_cj(acc[j], P[(size_t)j * n + k], b);  // rows >= it: below v_j's diagonal
         }
-        acc = wsum(acc);
-        if (lane == 0) Tl[j * nb + t] = acc;
+#pragma unroll
+        for (int j = 0; j < kQrMaxNb; ++j)
+          if (j < t) {
+            const cplx sj = wsum(acc[j]);
+            if (lane == 0) Tl[j * nb + t] = sj;
+          }
       }
       __syncthreads();
       // column t of T: thread j < t forms sum_{l=j}^{t-1} T(j, l) ip(l, t) from the inner products
@@ -186,14 +206,24 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
     for (int col = w + W * wave; col < n; col += W * nwaves) {
       if (lpos[col] < k0 + kb) continue;
       cplx* a = A + (size_t)col * n;
+      // the column's rows k0.. in registers (all loads in flight together)
+      cplx av[kQrMaxN / 64];
+#pragma unroll
+      for (int q = 0; q < kQrMaxN / 64; ++q) {
+        const int k = k0 + lane + 64 * q;
+        av[q] = k < n ? a[k] : mk(0, 0);
+      }
       cplx wv[kQrMaxNb];
 #pragma unroll
       for (int j = 0; j < kQrMaxNb; ++j) wv[j] = mk(0, 0);
-      for (int k = k0 + lane; k < n; k += 64) {
-        const cplx ak = a[k];
 #pragma unroll
-        for (int j = 0; j < kQrMaxNb; ++j)
-          if (j < kb) cfma_cj(wv[j], P[(size_t)j * n + k], ak);
+      for (int q = 0; q < kQrMaxN / 64; ++q) {
+        const int k = k0 + lane + 64 * q;
+        if (k < n) {
+#pragma unroll
+          for (int j = 0; j < kQrMaxNb; ++j)
+            if (j < kb) cfma_cj(wv[j], P[(size_t)j * n + k], av[q]);
+        }
       }
 #pragma unroll
       for (int j = 0; j < kQrMaxNb; ++j)
@@ -208,12 +238,16 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
           if (l <= j && j < kb) cfma_cj(acc, Tl[l * nb + j], wv[l]);
         z[j] = acc;
       }
-      for (int k = k0 + lane; k < n; k += 64) {
-        cplx ak = a[k];
 #pragma unroll
-        for (int j = 0; j < kQrMaxNb; ++j)
-          if (j < kb) ak = ak - P[(size_t)j * n + k] * z[j];
-        a[k] = ak;
+      for (int q = 0; q < kQrMaxN / 64; ++q) {
+        const int k = k0 + lane + 64 * q;
+        if (k < n) {
+          cplx ak = av[q];
+#pragma unroll
+          for (int j = 0; j < kQrMaxNb; ++j)
+            if (j < kb) ak = ak - P[(size_t)j * n + k] * z[j];
+          a[k] = ak;
+        }
       }
     }
     group_sync(g);
