@@ -77,7 +77,7 @@ struct ChainDesc {
   int* qr_perm;           // [n] logical -> physical column
   int* qr_lpos;           // [n] physical -> logical
   cplx* qr_tau;           // [n]
-  cplx* qr_Tf;            // [16 n] compact-WY T factors of the panels of the blocked QR
+  cplx* qr_Tf;            // [16 (n + 16)] compact-WY T factors of the panels of the blocked QR
   int* inv_piv;           // [n]
   cplx* inv_bcast;        // [2][n]
   cplx* lu_small;         // [32] delta (16) and M^-1 (16) of the last single proposal

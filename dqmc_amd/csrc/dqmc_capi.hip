@@ -207,7 +207,7 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
   ALLOC(d.qr_perm, int, n);
   ALLOC(d.qr_lpos, int, n);
   ALLOC(d.qr_tau, cplx, n);
-  ALLOC(d.qr_Tf, cplx, (size_t)16 * n);
+  ALLOC(d.qr_Tf, cplx, (size_t)16 * (n + 16));  // ceil(n / nb) panels of nb x nb, nb <= 16
   ALLOC(d.inv_piv, int, n);
   ALLOC(d.inv_bcast, cplx, 2 * n);
   ALLOC(d.lu_small, cplx, 32);
