@@ -212,11 +212,14 @@ hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, i
 // One local-update pass over all sites at `slice` (0-based). variant 1 updates G in global
 // memory after every accepted proposal, 2 keeps each workgroup's column block of G in LDS
 // (needs local_tile_fits), 3 delays the rank-4 updates and flushes them in groups (needs
-// local_delayed_fits).
+// local_delayed_fits), 4 processes the sites in windows with one group barrier per window
+// (needs local_window_sites(nmax, wg) >= 2: the largest window that fits in LDS; *tile
+// tells whether the workgroups' own columns of G stay in LDS as well).
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
                                 const int* slices, double box, int wg, int log, int variant);
 bool local_tile_fits(int nmax, int wg);
 bool local_delayed_fits(int nmax);
+int local_window_sites(int nmax, int wg, bool* tile = nullptr);
 
 // scal[S_LOGDET] = log|det UG| + sum log DG + log|det TG|  helper pieces
 hipError_t launch_sum_log(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, VecRef d, int slot,

@@ -76,7 +76,8 @@ struct Group {
   int cur = 0, dir = 1;  // reference current_slice (1-based), direction
   int wg_qr = 1, wg_inv = 1, wg_local = 1;
   int greens_route = 0;  // 0: D-scale separation, 1: verbatim reference route
-  int local_variant = 0;  // 0 auto, 1 G in global memory, 2 LDS-resident G tiles, 3 delayed updates
+  int local_variant = 0;  // 0 auto, 1 G in global memory, 2 LDS-resident G tiles, 3 delayed updates,
+                          // 4 windowed delayed updates
   int udt_route = 0;      // 0: blocked QR with pre-pivoting, 1: column-pivoted QR (zgeqp3 semantics)
   bool recomputed = false;
   // kernel-class timing (dqmc_profile)
@@ -312,7 +313,11 @@ int default_groups(Group& g) {
   int per = std::max(1, 256 / nch);
   g.wg_qr = std::max(1, std::min({per, g.n / 8, kMaxWG}));
   g.wg_inv = std::max(std::max(1, (g.n + 511) / 512), std::min({per, g.n / 8, kMaxWG}));
-  g.wg_local = std::max(1, std::min({per, g.n / 8, kMaxWG}));
+  // local updates: the windowed variant is fastest with many narrow column blocks (the
+  // own columns then fit in LDS beside the window); fall back to the n/8 blocks the other
+  // variants were tuned at when the window does not fit
+  g.wg_local = std::max(1, std::min({per, g.n / 4, kMaxWG}));
+  if (local_window_sites(g.n, g.wg_local) < 2) g.wg_local = std::max(1, std::min({per, g.n / 8, kMaxWG}));
   return 0;
 }
 
@@ -842,8 +847,10 @@ struct Ops {
     if (g.local_variant == 2 && !tile_fits)
       return fail(DQMC_E_ARG, "G tiles do not fit in LDS at this workgroup count");
     if (g.local_variant == 3 && !delay_fits) return fail(DQMC_E_ARG, "delayed updates do not fit in LDS at this size");
+    if (g.local_variant == 4 && local_window_sites(g.n, g.wg_local) < 2)
+      return fail(DQMC_E_ARG, "windowed updates do not fit in LDS at this size and workgroup count");
     int variant = g.local_variant;
-    if (variant == 0) variant = tile_fits ? 2 : delay_fits ? 3 : 1;
+    if (variant == 0) variant = local_window_sites(g.n, g.wg_local) >= 2 ? 4 : tile_fits ? 2 : delay_fits ? 3 : 1;
     LCHK(launch_local_updates(s(), d(), nc(), g.n, g.cur - 1, nullptr, box, g.wg_local, dolog, variant));
     return 0;
   }
@@ -1085,14 +1092,20 @@ int dqmc_ctx_destroy(dqmc_ctx* ctx) {
   return 0;
 }
 
+int dqmc_batch_set_groups(dqmc_batch* b, int wg_qr, int wg_inv, int wg_local) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  if (wg_qr > kMaxWG || wg_inv > kMaxWG || wg_local > kMaxWG) return fail(DQMC_E_ARG, "too many workgroups");
+  for (auto& g : b->groups) {
+    if (wg_qr > 0) g.wg_qr = wg_qr;
+    if (wg_inv > 0) g.wg_inv = std::max(wg_inv, (g.n + 511) / 512);
+    if (wg_local > 0) g.wg_local = wg_local;
+  }
+  return 0;
+}
+
 int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local) {
   CTX_CHECK(ctx);
-  Group& g = ctx->b->groups[0];
-  if (wg_qr > kMaxWG || wg_inv > kMaxWG || wg_local > kMaxWG) return fail(DQMC_E_ARG, "too many workgroups");
-  if (wg_qr > 0) g.wg_qr = wg_qr;
-  if (wg_inv > 0) g.wg_inv = std::max(wg_inv, (g.n + 511) / 512);
-  if (wg_local > 0) g.wg_local = wg_local;
-  return 0;
+  return dqmc_batch_set_groups(ctx->b, wg_qr, wg_inv, wg_local);
 }
 
 int dqmc_set_greens_route(dqmc_ctx* ctx, int route) {
@@ -1109,11 +1122,16 @@ int dqmc_set_udt_route(dqmc_ctx* ctx, int route) {
   return 0;
 }
 
+int dqmc_batch_set_local_variant(dqmc_batch* b, int variant) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  if (variant < 0 || variant > 4) return fail(DQMC_E_ARG, "local variant must be 0, 1, 2, 3 or 4");
+  for (auto& g : b->groups) g.local_variant = variant;
+  return 0;
+}
+
 int dqmc_set_local_variant(dqmc_ctx* ctx, int variant) {
   CTX_CHECK(ctx);
-  if (variant < 0 || variant > 3) return fail(DQMC_E_ARG, "local variant must be 0, 1, 2 or 3");
-  for (auto& g : ctx->b->groups) g.local_variant = variant;
-  return 0;
+  return dqmc_batch_set_local_variant(ctx->b, variant);
 }
 
 int dqmc_set_field(dqmc_ctx* ctx, const double* phi) {

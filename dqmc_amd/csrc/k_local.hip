@@ -168,13 +168,20 @@ __device__ inline cplx det4_laplace(const cplx* m) {
 __device__ inline cplx shfl_c(cplx v, int src) {
   return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64));
 }
+// DPP move of a complex value (quad_perm 0x00-0xFF, row_ror 0x121-0x12F, row_newbcast 0x150-0x15F)
+template <int CTRL>
+__device__ inline cplx dpp_c(cplx v) {
+  return mk(dpp_move<CTRL>(v.re), dpp_move<CTRL>(v.im));
+}
 
 // ---------------------------------------------------------------------------
 // Shared pieces of both local-update kernels
 // ---------------------------------------------------------------------------
 
 // LDS view of one slice: field of this and the two neighbouring slices, neighbour
-// table, the slice's uniforms and the old-field site blocks eV_op(phi_i, -1).
+// table, the slice's uniforms, the old-field site blocks eV_op(phi_i, -1), and per site
+// the sums of the field over its two time neighbours and its four space neighbours
+// (the latter kept current as sites are accepted).
 struct SliceLds {
   double* phl;  // [3N] current slice
   double* phe;  // [3N] earlier slice
@@ -182,10 +189,12 @@ struct SliceLds {
   int* nbl;     // [4N]
   double* ul;   // [4N]
   double* e1c;  // [4N] C, S.re, S.im, R
+  double* tnb;  // [3N] pla + phe
+  double* snb;  // [3N] sum over the four space neighbours of phl
   long long cur0, navail;
 };
 // doubles of LDS SliceLds takes (rounded up to keep 16-byte alignment after it)
-__host__ __device__ inline size_t slice_lds_doubles(int N) { return (size_t)19 * N + (N & 1); }
+__host__ __device__ inline size_t slice_lds_doubles(int N) { return (size_t)25 * N + (N & 1); }
 
 __device__ inline SliceLds load_slice(const ChainDesc& c, int l, double* smem) {
   const int N = c.N, M = c.M, tid = threadIdx.x;
@@ -196,6 +205,8 @@ __device__ inline SliceLds load_slice(const ChainDesc& c, int l, double* smem) {
   s.nbl = reinterpret_cast<int*>(s.pla + 3 * N);
   s.ul = smem + 11 * N;
   s.e1c = s.ul + 4 * N;
+  s.tnb = s.e1c + 4 * N;
+  s.snb = s.tnb + 3 * N;
   const int le = (l == 0) ? M - 1 : l - 1;  // time_neighbors (lattice.jl:144-153)
   const int la = (l == M - 1) ? 0 : l + 1;
   for (int e = tid; e < 3 * N; e += blockDim.x) {
@@ -204,6 +215,15 @@ __device__ inline SliceLds load_slice(const ChainDesc& c, int l, double* smem) {
     s.pla[e] = c.phi[(size_t)3 * N * la + e];
   }
   for (int e = tid; e < 4 * N; e += blockDim.x) s.nbl[e] = c.nb[e];
+  // neighbour sums in the order calc_boson_action_diff uses (action.jl:57-101), from
+  // global memory so no barrier is needed before them
+  for (int e = tid; e < 3 * N; e += blockDim.x) {
+    const int i = e / 3, a = e - 3 * i;
+    s.tnb[e] = c.phi[(size_t)3 * N * la + e] + c.phi[(size_t)3 * N * le + e];
+    double sn = 0.0;
+    for (int k = 0; k < 4; ++k) sn += c.phi[(size_t)3 * N * l + 3 * c.nb[k * N + i] + a];
+    s.snb[e] = sn;
+  }
   // at most 4 draws per proposal: this slice consumes no more than 4N uniforms
   s.cur0 = *c.ucursor;
   s.navail = min((long long)4 * N, c.n_uniforms - s.cur0);
@@ -233,20 +253,21 @@ struct Proposals {
 // Evaluate proposals from site p.i on, in wave 0 (every lane holds entry (ea, eb) of the 4x4
 // algebra; lanes 16..63 mirror 0..15), until one is accepted: its delta and M^-1 go to
 // sh.delta / sh.minv, the field of the site is updated in LDS and the site is returned.
-// Returns N at the end of the slice; sets *stop when the uniform stream ran out.
+// Sites from p.i up to limit - 1 are visited; returns limit when none of them is accepted
+// (p.i = limit) or when the uniform stream ran out (*stop set, p.i = the site it stopped at).
 // diag(j) returns this lane's entry G_jj(ea, eb) of the current G.
 // Reference: local_updates (local_updates.jl:1-39), calc_detratio (:42-59),
 // calc_boson_action_diff (action.jl:57-101).
 template <class Diag>
 __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposals& p, LocalShared& sh,
-                                  double box, int dolog, bool logger, Diag diag, int* stop) {
+                                  double box, int dolog, bool logger, Diag diag, int* stop, int limit) {
   const int N = c.N, lane = threadIdx.x & 63;
   const int ea = (lane & 15) >> 2, eb = lane & 3;
-  int t1[4], t2[4];  // site-block entry types of row ea (old operator) and column eb (new operator)
+  int t1[4], t2[4];  // entry types of row ea (old operator) and column (ea + eb) & 3 (new operator)
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     t1[k] = ev_type(ea * 4 + k);
-    t2[k] = ev_type(k * 4 + eb);
+    t2[k] = ev_type(k * 4 + ((ea + eb) & 3));
   }
   const double lam_dtau = c.lam * c.dtau;
   const double k_time = 1.0 / (c.dtau * c.c * c.c), k_space = 0.5 * c.dtau, k_dt = c.dtau;
@@ -255,10 +276,10 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
   *stop = 0;
   int i = p.i;
   cplx gnext = mk(0, 0);  // G_ii of the next site, loaded one site ahead
-  if (i < N) gnext = diag(i);
-  for (; i < N; ++i) {
+  if (i < limit) gnext = diag(i);
+  for (; i < limit; ++i) {
     const cplx gv = gnext;
-    if (i + 1 < N) gnext = diag(i + 1);
+    if (i + 1 < limit) gnext = diag(i + 1);
     if ((p.cur - s.cur0) + 4 > s.navail) {
       *stop = 1;
       break;
@@ -276,11 +297,8 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
     double dt_dot = 0.0, ds_dot = 0.0;
     for (int a = 0; a < 3; ++a) {
       const double diff = nw[a] - old[a];
-      const double tnb = s.pla[3 * i + a] + s.phe[3 * i + a];
-      double snb = 0.0;
-      for (int k = 0; k < 4; ++k) snb += s.phl[3 * s.nbl[k * N + i] + a];
-      dt_dot += tnb * diff;
-      ds_dot += snb * diff;
+      dt_dot += s.tnb[3 * i + a] * diff;
+      ds_dot += s.snb[3 * i + a] * diff;
     }
     double dS;
     if (!edrun) {
@@ -297,20 +315,39 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
     ev_coeffs_op(lam_dtau, nw[0], nw[1], nw[2], 1.0, C2, S2, R2);
     const double C1 = s.e1c[4 * i], R1 = s.e1c[4 * i + 3];
     const cplx S1 = mk(s.e1c[4 * i + 1], s.e1c[4 * i + 2]);
+    // this lane's entry of delta is (ea, (ea + eb) & 3): the skew lets the product below
+    // take both operands from DPP moves (quad broadcasts of delta, row rotations of I - G)
     cplx dm = mk(0, 0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) cfma(dm, ev_entry_t(C1, S1, R1, t1[k]), ev_entry_t(C2, S2, R2, t2[k]));
-    if (ea == eb) dm.re -= 1.0;
+    if (eb == 0) dm.re -= 1.0;
     const cplx gm = mk((ea == eb ? 1.0 : 0.0) - gv.re, -gv.im);
+    // mm(ea, eb) = sum_j delta(ea, (ea - j) & 3) (I - G)((ea - j) & 3, eb)
     cplx mm = mk(0, 0);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) cfma(mm, shfl_c(dm, ea * 4 + k), shfl_c(gm, k * 4 + eb));
+    cfma(mm, dpp_c<0x00>(dm), gm);
+    cfma(mm, dpp_c<0xFF>(dm), dpp_c<0x124>(gm));
+    cfma(mm, dpp_c<0xAA>(dm), dpp_c<0x128>(gm));
+    cfma(mm, dpp_c<0x55>(dm), dpp_c<0x12C>(gm));
     if (ea == eb) mm.re += 1.0;
-    cplx Mall[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) Mall[q] = shfl_c(mm, q);
+    cplx Mall[16];  // every lane: all of M, by row broadcasts of each lane of the 16-lane row
+    Mall[0] = dpp_c<0x150>(mm);
+    Mall[1] = dpp_c<0x151>(mm);
+    Mall[2] = dpp_c<0x152>(mm);
+    Mall[3] = dpp_c<0x153>(mm);
+    Mall[4] = dpp_c<0x154>(mm);
+    Mall[5] = dpp_c<0x155>(mm);
+    Mall[6] = dpp_c<0x156>(mm);
+    Mall[7] = dpp_c<0x157>(mm);
+    Mall[8] = dpp_c<0x158>(mm);
+    Mall[9] = dpp_c<0x159>(mm);
+    Mall[10] = dpp_c<0x15A>(mm);
+    Mall[11] = dpp_c<0x15B>(mm);
+    Mall[12] = dpp_c<0x15C>(mm);
+    Mall[13] = dpp_c<0x15D>(mm);
+    Mall[14] = dpp_c<0x15E>(mm);
+    Mall[15] = dpp_c<0x15F>(mm);
     const cplx det = det4_laplace(Mall);
-    if (fabs(det.im / det.re) > 1e-4) ++p.nonreal;
+    if (fabs(det.im) > 1e-4 * fabs(det.re)) ++p.nonreal;
     const double p_acc = exp_dS * det.re;
     int acc;
     double uval = -1.0;
@@ -336,6 +373,13 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
       p.dS_acc += -log(exp_dS);
       if (lane == 0)
         for (int a = 0; a < 3; ++a) s.phl[3 * i + a] = nw[a];
+      // the four neighbours' space sums, recomputed in the original order
+      if (lane < 12) {
+        const int j = s.nbl[(lane / 3) * N + i], a = lane - 3 * (lane / 3);
+        double sn = 0.0;
+        for (int k = 0; k < 4; ++k) sn += s.phl[3 * s.nbl[k * N + j] + a];
+        s.snb[3 * j + a] = sn;
+      }
       // M^-1 (b, a) = cofactor (a, b) / det M, one entry per lane
       cplx m3[9];
 #pragma unroll
@@ -346,7 +390,7 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
                  m3[2] * (m3[3] * m3[7] - m3[4] * m3[6]);
       if ((ea + eb) & 1) cof = -cof;
       if (lane < 16) {
-        sh.delta[lane] = dm;
+        sh.delta[ea * 4 + ((ea + eb) & 3)] = dm;
         sh.minv[eb * 4 + ea] = cof * cinv(det);
       }
       p.i = i + 1;
@@ -354,7 +398,7 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
     }
   }
   p.i = i;
-  return N;
+  return limit;
 }
 
 // counters and the updated field slice, written by workgroup 0
@@ -409,7 +453,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
       const int lane = tid & 63, ea = (lane & 15) >> 2, eb = lane & 3;
       const int site = evaluate_proposals(
           c, s, p, sh, box, dolog, w == 0, [&](int j) { return Gcur[(size_t)(j + eb * N) * n + (j + ea * N)]; },
-          &stop);
+          &stop, N);
       if (tid == 0) {
         sh.site = site;
         sh.stop = stop;
@@ -541,7 +585,7 @@ __global__ __launch_bounds__(256) void k_local_updates_tile(const ChainDesc* cha
       const int site = evaluate_proposals(
           c, s, p, sh, box, dolog, w == 0,
           [&](int j) { return from_g ? G0[(size_t)(j + eb * N) * n + (j + ea * N)] : dg[16 * j + 4 * ea + eb]; },
-          &st);
+          &st, N);
       if (tid == 0) {
         sh.site = site;
         sh.stop = st;
@@ -720,7 +764,7 @@ __global__ __launch_bounds__(256) void k_local_updates_delayed(const ChainDesc* 
       const int lane = tid & 63, ea = (lane & 15) >> 2, eb = lane & 3;
       int st;
       const int site = evaluate_proposals(c, s, p, sh, box, dolog, true,
-                                          [&](int j) { return Gd[16 * j + 4 * ea + eb]; }, &st);
+                                          [&](int j) { return Gd[16 * j + 4 * ea + eb]; }, &st, N);
       if (tid == 0) {
         sh.site = site;
         sh.stop = st;
@@ -820,6 +864,242 @@ __global__ __launch_bounds__(256) void k_local_updates_delayed(const ChainDesc* 
   finish_slice(c, l, s, p, dolog, stop);
 }
 
+// ---------------------------------------------------------------------------
+// Variant 4: windowed delayed updates. The sites of the slice are split into R windows
+// (R even, at most K sites each). Within a window every workgroup keeps the accepted
+// updates as factor pairs in LDS -- A_b = (G'[:, c] - E) M^-1 (n x 4, all rows) and
+// B_b = delta G'[r, :] restricted to its own columns -- together with the current 4x4
+// blocks G'[r_j, c_k] between the window's sites, so decisions and new factor pairs
+// need no communication. At the end of a window each workgroup writes its columns of
+// G + sum_b A_b B_b into the partner buffer (M_G and M_GALT alternate; R is even, so the
+// slice ends in M_G), then one group barrier: one barrier per window instead of one per
+// accepted proposal. Global G is read only from the current window's buffer, which no
+// workgroup writes during the window. With TILE the workgroup's own columns stay in
+// LDS for the whole slice, so the end-of-window pass reads nothing from global memory.
+// Same arithmetic as update_greens! (local_updates.jl:61-95) up to the order of sums.
+// ---------------------------------------------------------------------------
+constexpr int kWinMax = 8;
+constexpr size_t kWindowLdsMax = 160 * 1024 - 1024;  // the CU's LDS less this kernel's static part
+
+__host__ __device__ inline int window_bo_stride(int n, int wg) { return (n + wg - 1) / wg + 1; }
+__host__ __device__ inline size_t window_lds_bytes(int n, int N, int wg, int K, bool tile) {
+  const size_t ncs = window_bo_stride(n, wg);
+  const size_t t = tile ? (size_t)(ncs - 1) * n : 0;
+  return slice_lds_doubles(N) * 8 + (t + (size_t)K * n * 4 + (size_t)K * 4 * ncs + (size_t)2 * K * K * 16) * 16;
+}
+
+template <bool TILE>
+__global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* chains, int slice, const int* slices,
+                                                              double box, int dolog, int K) {
+  const ChainDesc& c = chains[blockIdx.y];
+  const int n = c.n, N = c.N;
+  const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
+  const int l = slices ? slices[blockIdx.y] : slice;
+  GroupSync g = chain_group(c);
+  const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
+  const int ncol = cb1 - cb0;
+  const int ncs = window_bo_stride(n, W);
+
+  extern __shared__ double smem[];
+  const SliceLds s = load_slice(c, l, smem);
+  cplx* T = reinterpret_cast<cplx*>(smem + slice_lds_doubles(N));  // [ncs - 1][n] own columns (TILE)
+  cplx* Aw = T + (TILE ? (size_t)(ncs - 1) * n : 0);                 // [K][4][n]  A_b (rows contiguous)
+  cplx* Bo = Aw + (size_t)K * n * 4;                                 // [K][4][ncs] B_b, own columns
+  cplx* Gp = Bo + (size_t)K * 4 * ncs;                               // [K][K][16] G'[r_j, c_k]
+  cplx* Bw = Gp + (size_t)K * K * 16;                                // [K][K][16] B_b[:, c_k]
+  __shared__ LocalShared sh;
+  cplx* X[2] = {c.mat[M_G], c.mat[M_GALT]};
+  int R = (N + K - 1) / K;
+  R += R & 1;
+  if (TILE) {
+    const cplx* G0 = X[0] + (size_t)cb0 * n;
+    const int tot = ncol * n;
+    for (int e0 = tid; e0 < tot; e0 += 4 * blockDim.x) {
+      cplx v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = G0[min(e0 + q * (int)blockDim.x, tot - 1)];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (e0 + q * (int)blockDim.x < tot) {
+          T[e0 + q * blockDim.x].re = v[q].re;  // (whole-struct copies from a register array
+          T[e0 + q * blockDim.x].im = v[q].im;  // are lowered through a stack slot)
+        }
+    }
+  }
+
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  int stop = 0;  // the uniform stream ran out: later windows make no proposals
+  for (int rw = 0; rw < R; ++rw) {
+    const int s0 = (int)((long long)rw * N / R), s1 = (int)((long long)(rw + 1) * N / R), kw = s1 - s0;
+    const cplx* Gc = X[rw & 1];
+    cplx* Gn = X[(rw + 1) & 1];
+    // Gp[j][k](a, b) = G(r_j + a N, c_k + b N) for the window's sites j, k
+    for (int e = tid; e < kw * kw * 16; e += blockDim.x) {
+      const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
+      Gp[(jj * K + kk) * 16 + a * 4 + b] = Gc[(size_t)(s0 + kk + b * N) * n + (s0 + jj + a * N)];
+    }
+    __syncthreads();
+    int m = 0;
+    while (!stop) {
+      if (tid < 64) {
+        const int lane = tid & 63, ea = (lane & 15) >> 2, eb = lane & 3;
+        int st;
+        const int site = evaluate_proposals(
+            c, s, p, sh, box, dolog, w == 0,
+            [&](int j) { return Gp[((j - s0) * (K + 1)) * 16 + ea * 4 + eb]; }, &st, s1);
+        if (tid == 0) {
+          sh.site = site;
+          sh.stop = st;
+        }
+      }
+      __syncthreads();
+      const int site = sh.site;
+      stop = sh.stop;
+      if (site >= s1) break;
+      const int ii = site - s0;
+      cplx* Am = Aw + (size_t)m * n * 4;
+      // A = (G'[:, c_i] - E) M^-1 with G'[:, c_i] = G[:, c_i] + sum_b A_b B_b[:, c_i];
+      // two rows per thread, their eight loads issued together
+      for (int r0 = tid; r0 < n; r0 += 2 * blockDim.x) {
+        const int r1 = min(r0 + (int)blockDim.x, n - 1);
+        cplx g4[2][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          g4[0][k] = Gc[(size_t)(site + k * N) * n + r0];
+          g4[1][k] = Gc[(size_t)(site + k * N) * n + r1];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = h ? r1 : r0;
+          for (int b = 0; b < m; ++b) {
+            const cplx* ab = Aw + (size_t)b * 4 * n + r;
+            const cplx* bw = Bw + (b * K + ii) * 16;
+            const cplx a0 = ab[0], a1 = ab[n], a2 = ab[2 * n], a3 = ab[3 * n];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              cfma(g4[h][k], a0, bw[k]);
+              cfma(g4[h][k], a1, bw[4 + k]);
+              cfma(g4[h][k], a2, bw[8 + k]);
+              cfma(g4[h][k], a3, bw[12 + k]);
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (r == site + k * N) g4[h][k].re -= 1.0;
+          if (h == 0 || r0 + (int)blockDim.x < n) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              cplx acc = mk(0, 0);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) cfma(acc, g4[h][q], sh.minv[q * 4 + k]);
+              Am[(size_t)k * n + r] = acc;
+            }
+          }
+        }
+      }
+      // B = delta G'[r_i, own columns] with G'[r_i, :] = G[r_i, :] + sum_b A_b[r_i, :] B_b
+      for (int jl = tid; jl < ncol; jl += blockDim.x) {
+        cplx g4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          g4[q] = TILE ? T[(size_t)jl * n + site + q * N] : Gc[(size_t)(cb0 + jl) * n + site + q * N];
+        for (int b = 0; b < m; ++b) {
+          cplx bo[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) bo[k] = Bo[((size_t)b * 4 + k) * ncs + jl];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const cplx* ab = Aw + (size_t)b * 4 * n + site + q * N;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) cfma(g4[q], ab[(size_t)k * n], bo[k]);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          cplx acc = mk(0, 0);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cfma(acc, sh.delta[k * 4 + q], g4[q]);
+          Bo[((size_t)m * 4 + k) * ncs + jl] = acc;
+        }
+      }
+      // B[:, c_k] for the window's sites: delta G'[r_i, c_k]
+      for (int e = tid; e < kw * 16; e += blockDim.x) {
+        const int kk = e >> 4, k = (e >> 2) & 3, b = e & 3;
+        cplx acc = mk(0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cfma(acc, sh.delta[k * 4 + q], Gp[(ii * K + kk) * 16 + q * 4 + b]);
+        Bw[(m * K + kk) * 16 + k * 4 + b] = acc;
+      }
+      __syncthreads();
+      // the blocks of the sites still ahead: G'[r_j, c_k] += A[r_j, :] B[:, c_k]
+      for (int e = tid; e < kw * kw * 16; e += blockDim.x) {
+        const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
+        if (jj <= ii) continue;
+        const cplx* ar = Am + s0 + jj + a * N;
+        cplx acc = Gp[(jj * K + kk) * 16 + a * 4 + b];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cfma(acc, ar[(size_t)q * n], Bw[(m * K + kk) * 16 + q * 4 + b]);
+        Gp[(jj * K + kk) * 16 + a * 4 + b] = acc;
+      }
+      ++m;
+      __syncthreads();
+    }
+    // own columns of the partner buffer: G + sum_b A_b B_b (a plain copy when m = 0)
+    for (int r = tid; r < n; r += blockDim.x) {
+      for (int j0 = 0; j0 < ncol; j0 += 4) {
+        cplx acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int jl = min(j0 + t, ncol - 1);
+          acc[t] = TILE ? T[(size_t)jl * n + r] : Gc[(size_t)(cb0 + jl) * n + r];
+        }
+        for (int b = 0; b < m; ++b) {
+          const cplx* ab = Aw + (size_t)b * 4 * n + r;
+          const cplx a0 = ab[0], a1 = ab[n], a2 = ab[2 * n], a3 = ab[3 * n];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int jl = min(j0 + t, ncol - 1);
+            cfma(acc[t], a0, Bo[((size_t)b * 4 + 0) * ncs + jl]);
+            cfma(acc[t], a1, Bo[((size_t)b * 4 + 1) * ncs + jl]);
+            cfma(acc[t], a2, Bo[((size_t)b * 4 + 2) * ncs + jl]);
+            cfma(acc[t], a3, Bo[((size_t)b * 4 + 3) * ncs + jl]);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          if (j0 + t < ncol) {
+            if (TILE) {
+              T[(size_t)(j0 + t) * n + r].re = acc[t].re;
+              T[(size_t)(j0 + t) * n + r].im = acc[t].im;
+            }
+            cplx* d = Gn + (size_t)(cb0 + j0 + t) * n + r;
+            d->re = acc[t].re;
+            d->im = acc[t].im;
+          }
+      }
+    }
+    if (rw + 1 < R) {
+      group_sync(g);
+      if (group_failed(g)) return;
+    }
+  }
+  if (w == 0) finish_slice(c, l, s, p, dolog, stop);
+}
+
+// the window of variant 4 at this shape: its K, and whether the own columns fit in LDS too
+// (preferred when the window keeps at least 3 sites); K = 0 when nothing fits
+int local_window_sites(int nmax, int wg, bool* tile) {
+  for (int K = kWinMax; K >= 3; --K)
+    if (window_lds_bytes(nmax, nmax / 4, wg, K, true) <= kWindowLdsMax) {
+      if (tile) *tile = true;
+      return K;
+    }
+  if (tile) *tile = false;
+  for (int K = kWinMax; K >= 2; --K)
+    if (window_lds_bytes(nmax, nmax / 4, wg, K, false) <= kWindowLdsMax) return K;
+  return 0;
+}
+
 bool local_delayed_fits(int nmax) { return delayed_lds_bytes(nmax, nmax / 4) <= 150 * 1024; }
 
 // LDS bytes of the two variants at the given shape (dynamic part)
@@ -837,20 +1117,28 @@ bool local_tile_fits(int nmax, int wg) { return local_lds_tile(nmax, nmax / 4, w
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
                                 const int* slices, double box, int wg, int log, int variant) {
   const int Nmax = nmax / 4;
+  if (variant < 1 || variant > 4) return hipErrorInvalidValue;
+  bool tile = false;
+  int K = variant == 4 ? local_window_sites(nmax, wg, &tile) : 0;
+  if (variant == 4 && K < 2) return hipErrorInvalidValue;
+  const int slot = variant == 4 && tile ? 5 : variant;
   const void* fn = variant == 2   ? (const void*)k_local_updates_tile
                    : variant == 3 ? (const void*)k_local_updates_delayed
+                   : variant == 4 ? (tile ? (const void*)k_local_updates_window<true>
+                                          : (const void*)k_local_updates_window<false>)
                                   : (const void*)k_local_updates;
   const size_t lds = variant == 2   ? local_lds_tile(nmax, Nmax, wg)
                      : variant == 3 ? delayed_lds_bytes(nmax, Nmax)
+                     : variant == 4 ? window_lds_bytes(nmax, Nmax, wg, K, tile)
                                     : local_lds_global(nmax, Nmax, wg);
-  if (lds > kLocalLdsMax) return hipErrorInvalidValue;
-  static size_t attr[4] = {0, 0, 0, 0};
-  if (lds > 65536 && lds > attr[variant & 3]) {
+  if (lds > (variant == 4 ? kWindowLdsMax : kLocalLdsMax)) return hipErrorInvalidValue;
+  static size_t attr[6] = {0, 0, 0, 0, 0, 0};
+  if (lds > 65536 && lds > attr[slot]) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    attr[variant & 3] = lds;
+    attr[slot] = lds;
   }
-  void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log};
+  void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log, (void*)&K};
   hipError_t e = hipLaunchCooperativeKernel(fn, dim3(wg, nchains), dim3(256), args, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
   return e;

@@ -204,7 +204,8 @@ class DQMC:
         check(self.lib.dqmc_set_udt_route(self.h, route))
 
     def set_local_variant(self, variant):
-        """0 auto (default); 1 G in global memory; 2 LDS-resident G tiles; 3 delayed updates."""
+        """0 auto (default); 1 G in global memory; 2 LDS-resident G tiles; 3 delayed updates;
+        4 windowed delayed updates."""
         check(self.lib.dqmc_set_local_variant(self.h, variant))
 
     def set_greens_route(self, route):
@@ -478,6 +479,14 @@ class Batch:
 
     def synchronize(self):
         check(self.lib.dqmc_batch_synchronize(self.h))
+
+    def set_groups(self, wg_qr=0, wg_inv=0, wg_local=0):
+        """Workgroups per chain of every shape group (0 keeps the current value)."""
+        check(self.lib.dqmc_batch_set_groups(self.h, wg_qr, wg_inv, wg_local))
+
+    def set_local_variant(self, variant):
+        """Local-update variant of every shape group (see DQMC.set_local_variant)."""
+        check(self.lib.dqmc_batch_set_local_variant(self.h, variant))
 
     def greens(self, chain):
         n = self.params[chain].n

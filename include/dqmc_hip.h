@@ -106,9 +106,11 @@ int dqmc_set_greens_route(dqmc_ctx* ctx, int route);
 /* how local_updates (local_updates.jl:1-39) applies update_greens! while it sweeps the
  * sites of a slice: 1 updates G in global memory after every accepted proposal; 2 keeps
  * each workgroup's column block of G in LDS; 3 delays the rank-4 updates (up to 8 pending,
- * decisions from LDS-resident diagonal blocks) and adds them to G in one pass; 0 (default)
- * picks 2, else 3, else 1, by what fits in LDS. Same results to rounding; an error when a
- * forced variant does not fit. */
+ * decisions from LDS-resident diagonal blocks) and adds them to G in one pass; 4 visits the
+ * sites in windows of up to 8, keeping the window's accepted updates as factor pairs in
+ * every workgroup and adding them to G once per window (one group barrier per window);
+ * 0 (default) picks 4, else 2, else 3, else 1, by what fits in LDS. Same results to
+ * rounding; an error when a forced variant does not fit. */
 int dqmc_set_local_variant(dqmc_ctx* ctx, int variant);
 /* how the stack's decompose_udt! (linalg.jl:20-39) factors inside build_stack /
  * add_slice_sequence_*: 0 (default) blocked Householder QR after pre-pivoting the
@@ -246,6 +248,9 @@ int dqmc_batch_get_greens(dqmc_batch* b, int chain, double* G);
 int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st);
 /* capture each shape group's sweep in a HIP graph and replay it (1) or launch eagerly (0) */
 int dqmc_batch_use_graph(dqmc_batch* b, int enable);
+/* dqmc_ctx_set_groups / dqmc_set_local_variant for every shape group of a batch */
+int dqmc_batch_set_groups(dqmc_batch* b, int wg_qr, int wg_inv, int wg_local);
+int dqmc_batch_set_local_variant(dqmc_batch* b, int variant);
 
 /* ---- kernel timing -------------------------------------------------------
  * No reference counterpart (the reference's TimerOutputs sections,

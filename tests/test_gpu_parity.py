@@ -349,14 +349,18 @@ def test_seeded_chain_small(amd):
     c2.close()
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 def test_local_update_variants(amd, variant):
-    """G in global memory (1), G tiles resident in LDS (2) and delayed updates (3): same
-    decisions as the oracle, G within 1e-10, for several workgroup counts (L=6, beta=2)."""
+    """G in global memory (1), G tiles resident in LDS (2), delayed updates (3) and windowed
+    delayed updates (4): same decisions as the oracle, G within 1e-10, for several workgroup
+    counts (L=6 and 8, beta=2)."""
     mod, DQMC = amd
-    for wg in (3, 8, 36):
-        _say("local-update variant %d, %d workgroups, L=6 M=20, 3 sweeps" % (variant, wg))
-        p = config_params(mod, 6, 20)
+    cases = [(6, 3), (6, 8), (6, 36)]
+    if variant == 4:
+        cases.append((8, 4))  # n = 256 at 4 workgroups: the window without LDS-resident columns
+    for L, wg in cases:
+        _say("local-update variant %d, %d workgroups, L=%d M=20, 3 sweeps" % (variant, wg, L))
+        p = config_params(mod, L, 20)
 
         def setup(c):
             c.set_groups(0, 0, wg)
