@@ -89,10 +89,12 @@ def cpu_baseline(p_amd, phi, slices=100):
 
 def batch_measure(amd, nchains, sweeps, warmup):
     """Aggregate sweeps/s of `nchains` independent chains at the same workload (L=10, beta=20),
-    advanced together by one batch (every kernel launch covers all chains)."""
+    advanced together by one batch (every kernel launch covers all chains; sweeps replayed
+    from HIP graphs)."""
     from dqmc_amd import Batch
     params = [workload_params(amd, seed=2000 + c) for c in range(nchains)]
     b = Batch(params)
+    b.use_graph(True)  # each chain group's sweep is replayed from a captured HIP graph
     confs = [np.random.default_rng(p.seed).uniform(-1.0, 1.0, (3, p.N, p.slices)) for p in params]
     b.init(confs)
     if warmup > 0:

@@ -80,6 +80,18 @@ struct Group {
                           // 4 windowed delayed updates
   int udt_route = 0;      // 0: blocked QR with pre-pivoting, 1: column-pivoted QR (zgeqp3 semantics)
   bool recomputed = false;
+  // one sweep (M update steps) captured as a HIP graph per starting state and settings
+  struct SweepGraph {
+    int cur0, dir0, dolog;
+    double box;
+    int cur1, dir1;
+    hipGraphExec_t exec;
+  };
+  std::vector<SweepGraph> graphs;
+  void drop_graphs() {
+    for (auto& e : graphs) (void)hipGraphExecDestroy(e.exec);
+    graphs.clear();
+  }
   // kernel-class timing (dqmc_profile)
   unsigned prof_mask = 0;
   struct Rec {
@@ -931,6 +943,7 @@ int batch_destroy(dqmc_batch* b) {
   if (!b) return 0;
   (void)hipDeviceSynchronize();
   for (auto& g : b->groups) {
+    g.drop_graphs();
     if (g.stream) (void)hipStreamDestroy(g.stream);
     if (g.d_desc) (void)hipFree(g.d_desc);
   }
@@ -1005,6 +1018,47 @@ int group_sweeps(Group& g, int nsweeps, double box) {
       if ((rc = o.propagate())) return rc;
       if ((rc = o.local_updates(box, 0))) return rc;
     }
+  return 0;
+}
+
+// One sweep of a group through a HIP graph. The work of a sweep is fixed by the group's
+// (current_slice, direction) at its start, the box and the logging flag -- propagate is a
+// host-side state machine and every kernel reads the chains' state from device memory -- so
+// each starting state is captured once and replayed; the host state then jumps to where
+// the captured sweep ends.
+int group_sweep_graph(Group& g, double box, int dolog) {
+  for (auto& e : g.graphs)
+    if (e.cur0 == g.cur && e.dir0 == g.dir && e.dolog == dolog && e.box == box) {
+      HIPCHK(hipGraphLaunch(e.exec, g.stream));
+      g.cur = e.cur1;
+      g.dir = e.dir1;
+      return 0;
+    }
+  Group::SweepGraph e{g.cur, g.dir, dolog, box, 0, 0, nullptr};
+  HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeRelaxed));
+  int rc = 0;
+  {
+    Ops o(g);
+    for (int step = 0; step < g.M && !rc; ++step) {
+      rc = o.propagate();
+      if (!rc) rc = o.local_updates(box, dolog);
+    }
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(g.stream, &graph);
+  hipError_t ie = hipSuccess;
+  if (!rc && ce == hipSuccess) ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc || ce != hipSuccess || ie != hipSuccess) {
+    g.cur = e.cur0;  // nothing of the sweep ran
+    g.dir = e.dir0;
+    if (rc) return rc;
+    return fail(DQMC_E_HIP, std::string("sweep graph: ") + hipGetErrorString(ce != hipSuccess ? ce : ie));
+  }
+  e.cur1 = g.cur;
+  e.dir1 = g.dir;
+  g.graphs.push_back(e);
+  HIPCHK(hipGraphLaunch(e.exec, g.stream));
   return 0;
 }
 
@@ -1095,7 +1149,9 @@ int dqmc_ctx_destroy(dqmc_ctx* ctx) {
 int dqmc_batch_set_groups(dqmc_batch* b, int wg_qr, int wg_inv, int wg_local) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
   if (wg_qr > kMaxWG || wg_inv > kMaxWG || wg_local > kMaxWG) return fail(DQMC_E_ARG, "too many workgroups");
+  RC(batch_sync(b));
   for (auto& g : b->groups) {
+    g.drop_graphs();
     if (wg_qr > 0) g.wg_qr = wg_qr;
     if (wg_inv > 0) g.wg_inv = std::max(wg_inv, (g.n + 511) / 512);
     if (wg_local > 0) g.wg_local = wg_local;
@@ -1111,6 +1167,8 @@ int dqmc_ctx_set_groups(dqmc_ctx* ctx, int wg_qr, int wg_inv, int wg_local) {
 int dqmc_set_greens_route(dqmc_ctx* ctx, int route) {
   CTX_CHECK(ctx);
   if (route != 0 && route != 1) return fail(DQMC_E_ARG, "route must be 0 or 1");
+  RC(batch_sync(ctx->b));
+  ctx->b->groups[0].drop_graphs();
   ctx->b->groups[0].greens_route = route;
   return 0;
 }
@@ -1118,14 +1176,22 @@ int dqmc_set_greens_route(dqmc_ctx* ctx, int route) {
 int dqmc_set_udt_route(dqmc_ctx* ctx, int route) {
   CTX_CHECK(ctx);
   if (route != 0 && route != 1) return fail(DQMC_E_ARG, "udt route must be 0 or 1");
-  for (auto& g : ctx->b->groups) g.udt_route = route;
+  RC(batch_sync(ctx->b));
+  for (auto& g : ctx->b->groups) {
+    g.drop_graphs();
+    g.udt_route = route;
+  }
   return 0;
 }
 
 int dqmc_batch_set_local_variant(dqmc_batch* b, int variant) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
   if (variant < 0 || variant > 4) return fail(DQMC_E_ARG, "local variant must be 0, 1, 2, 3 or 4");
-  for (auto& g : b->groups) g.local_variant = variant;
+  RC(batch_sync(b));
+  for (auto& g : b->groups) {
+    g.drop_graphs();
+    g.local_variant = variant;
+  }
   return 0;
 }
 
@@ -1709,6 +1775,17 @@ int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box) {
     for (int ci : g.chains)
       if (box[ci] != bx) return fail(DQMC_E_ARG, "chains of one shape group must share the box size");
   }
+  bool graphs = b->use_graph;
+  for (auto& g : b->groups) graphs = graphs && g.prof_mask == 0;  // timing events stay eager
+  if (graphs) {
+    for (int sw = 0; sw < nsweeps; ++sw)
+      for (auto& g : b->groups) {
+        int dolog = 0;
+        for (int ci : g.chains) dolog |= b->chains[ci].log_cap > 0;
+        RC(group_sweep_graph(g, box[g.chains[0]], dolog));
+      }
+    return 0;
+  }
   std::vector<int> steps(b->groups.size(), 0);
   bool more = true;
   while (more) {
@@ -1784,8 +1861,16 @@ int dqmc_profile_read(dqmc_ctx* ctx, double* ms, long long* count, int nclass) {
 
 int dqmc_batch_use_graph(dqmc_batch* b, int enable) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
+  RC(batch_sync(b));
   b->use_graph = enable != 0;
+  if (!b->use_graph)
+    for (auto& g : b->groups) g.drop_graphs();
   return 0;
+}
+
+int dqmc_use_graph(dqmc_ctx* ctx, int enable) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_use_graph(ctx->b, enable);
 }
 
 }  // extern "C"

@@ -208,6 +208,11 @@ class DQMC:
         4 windowed delayed updates."""
         check(self.lib.dqmc_set_local_variant(self.h, variant))
 
+    def use_graph(self, enable=True):
+        """Capture each sweep's kernel sequence in a HIP graph (one per starting state) and
+        replay it; same results as eager launches."""
+        check(self.lib.dqmc_use_graph(self.h, 1 if enable else 0))
+
     def set_greens_route(self, route):
         """0: D-scale separation (default); 1: the reference's calculate_greens verbatim."""
         check(self.lib.dqmc_set_greens_route(self.h, route))
@@ -487,6 +492,10 @@ class Batch:
     def set_local_variant(self, variant):
         """Local-update variant of every shape group (see DQMC.set_local_variant)."""
         check(self.lib.dqmc_batch_set_local_variant(self.h, variant))
+
+    def use_graph(self, enable=True):
+        """Capture each shape group's sweep in a HIP graph and replay it (see DQMC.use_graph)."""
+        check(self.lib.dqmc_batch_use_graph(self.h, 1 if enable else 0))
 
     def greens(self, chain):
         n = self.params[chain].n

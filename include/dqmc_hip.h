@@ -246,8 +246,14 @@ int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box);
 int dqmc_batch_synchronize(dqmc_batch* b);
 int dqmc_batch_get_greens(dqmc_batch* b, int chain, double* G);
 int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st);
-/* capture each shape group's sweep in a HIP graph and replay it (1) or launch eagerly (0) */
+/* capture each shape group's sweep in a HIP graph and replay it (1) or launch eagerly (0).
+ * No reference counterpart (the reference has no device). A sweep's kernel sequence is fixed
+ * by the group's (current_slice, direction) at its start, the box and whether proposals are
+ * logged, so one graph is captured per such state (two for alternating up/down sweeps) on
+ * first use and replayed after; changing workgroup counts or routes drops the graphs. Same
+ * results as eager launches; not used while kernel timing (dqmc_batch_profile) is on. */
 int dqmc_batch_use_graph(dqmc_batch* b, int enable);
+int dqmc_use_graph(dqmc_ctx* ctx, int enable);
 /* dqmc_ctx_set_groups / dqmc_set_local_variant for every shape group of a batch */
 int dqmc_batch_set_groups(dqmc_batch* b, int wg_qr, int wg_inv, int wg_local);
 int dqmc_batch_set_local_variant(dqmc_batch* b, int variant);
