@@ -12,7 +12,10 @@ of 64 independent chains of the same workload advanced together) and "grad" (dS/
 evaluations per second at L=12, beta=10).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch C]
-                       [--batch-sweeps S] [--grad-evals E]
+                       [--batch-sweeps S] [--grad-evals E] [--no-tune]
+
+Both sweep legs run the autotuner (dqmc_autotune) after init, outside the timed region;
+its choice is reported under "tuning".
 """
 import argparse
 import json
@@ -87,7 +90,7 @@ def cpu_baseline(p_amd, phi, slices=100):
                                                                                    slices // p.safe_mult)}
 
 
-def batch_measure(amd, nchains, sweeps, warmup):
+def batch_measure(amd, nchains, sweeps, warmup, tune=True):
     """Aggregate sweeps/s of `nchains` independent chains at the same workload (L=10, beta=20),
     advanced together by one batch (every kernel launch covers all chains; sweeps replayed
     from HIP graphs)."""
@@ -97,6 +100,7 @@ def batch_measure(amd, nchains, sweeps, warmup):
     b.use_graph(True)  # each chain group's sweep is replayed from a captured HIP graph
     confs = [np.random.default_rng(p.seed).uniform(-1.0, 1.0, (3, p.N, p.slices)) for p in params]
     b.init(confs)
+    tuning = b.autotune(reps=2) if tune else None
     if warmup > 0:
         b.sweep(warmup)
     b.synchronize()
@@ -109,7 +113,8 @@ def batch_measure(amd, nchains, sweeps, warmup):
     return {"chains": nchains, "value": nchains * sweeps / dt, "unit": "sweeps/s (all chains)",
             "per_chain": sweeps / dt, "sweeps": sweeps, "warmup": warmup, "seconds": dt,
             "drift_max": max(st.drift_max for st in sts),
-            "acceptance": sum(st.accepted for st in sts) / max(sum(st.proposed for st in sts), 1)}
+            "acceptance": sum(st.accepted for st in sts) / max(sum(st.proposed for st in sts), 1),
+            "tuning": tuning}
 
 
 def grad_measure(amd, evals, warmup):
@@ -141,6 +146,8 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=100)
     ap.add_argument("--batch", type=int, default=64, help="chains of the batch leg (0 skips it)")
     ap.add_argument("--batch-sweeps", type=int, default=1)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (kernel profiles use this so tuning launches stay out)")
     ap.add_argument("--grad-evals", type=int, default=3, help="0 skips the gradient leg")
     ap.add_argument("--pmc-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known traffic that "
@@ -168,6 +175,7 @@ def main():
     phi = rng.uniform(-1.0, 1.0, (3, p.N, p.slices))
     chain = DQMC(p, uniforms=UniformStream(p.seed + 7))
     chain.init(phi)
+    tuning = None if args.no_tune else chain.autotune(reps=2)
     # warmup with every kernel class timed, to find the dominant kernel
     chain.lib.dqmc_profile(chain.h, (1 << _capi.K_NCLASS) - 1)
     if args.warmup > 0:
@@ -226,10 +234,11 @@ def main():
         "drift_max": st.drift_max,
         "acceptance": st.accepted / max(st.proposed, 1),
         "kernel_ms_per_sweep_warmup": warm_breakdown,
+        "tuning": tuning,
     }
     chain.close()
     if args.batch > 0:
-        out["batch"] = batch_measure(dqmc_amd, args.batch, args.batch_sweeps, 1)
+        out["batch"] = batch_measure(dqmc_amd, args.batch, args.batch_sweeps, 1, tune=not args.no_tune)
     if args.grad_evals > 0:
         out["grad"] = grad_measure(dqmc_amd, args.grad_evals, 1)
     if not args.no_cpu:

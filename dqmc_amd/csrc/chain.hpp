@@ -200,11 +200,13 @@ hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, i
                           MatRef Qout, MatRef Tout);
 
 // Blocked Householder QR of A (in place, k_qrp layout; D = |Re R_kk|); prepivot = 1 orders
-// the columns by decreasing initial norm first, 0 keeps them.
+// the columns by decreasing initial norm first, 0 keeps them. nb: panel width, 16 (n <= 512)
+// or 8; 0 picks 16 up to n = 512, else 8.
 hipError_t launch_qr_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
-                             int prepivot, int wg);
-// B <- R^-1 Q^H B with the factors of launch_qr_blocked(prepivot = 0) in A
-hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B);
+                             int prepivot, int wg, int nb = 0);
+// B <- R^-1 Q^H B with the factors of launch_qr_blocked(prepivot = 0) in A (same nb)
+hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B,
+                           int nb = 0);
 
 // X <- X^-1 in place (Gauss-Jordan, partial pivoting); scal[S_LOGDET_A] = log|det X|.
 hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <random>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -79,6 +80,7 @@ struct Group {
   int local_variant = 0;  // 0 auto, 1 G in global memory, 2 LDS-resident G tiles, 3 delayed updates,
                           // 4 windowed delayed updates
   int udt_route = 0;      // 0: blocked QR with pre-pivoting, 1: column-pivoted QR (zgeqp3 semantics)
+  int qr_nb = 0;          // panel width of the blocked QR (0: 16 up to n = 512, else 8)
   bool recomputed = false;
   // one sweep (M update steps) captured as a HIP graph per starting state and settings
   struct SweepGraph {
@@ -330,6 +332,9 @@ int default_groups(Group& g) {
   // variants were tuned at when the window does not fit
   g.wg_local = std::max(1, std::min({per, g.n / 4, kMaxWG}));
   if (local_window_sites(g.n, g.wg_local) < 2) g.wg_local = std::max(1, std::min({per, g.n / 8, kMaxWG}));
+  // a lone chain's QR is latency-bound: the narrower panel shortens each panel's serial part
+  // (the autotuner measures this per shape); many chains share the chip better with 16
+  g.qr_nb = nch <= 8 ? 8 : 0;
   return 0;
 }
 
@@ -391,7 +396,7 @@ struct Ops {
     {
       ProfScope ps(g, DQMC_K_QRP);
       if (stack && g.udt_route == 0)
-        LCHK(launch_qr_blocked(s(), d(), nc(), g.n, A, D, 1, g.wg_qr));
+        LCHK(launch_qr_blocked(s(), d(), nc(), g.n, A, D, 1, g.wg_qr, g.qr_nb));
       else
         LCHK(launch_qrp(s(), d(), nc(), g.n, A, D, g.wg_qr));
     }
@@ -528,10 +533,10 @@ struct Ops {
       // [.] = Q R (Householder, no pivoting: the matrix is well conditioned), then
       // Y = [.]^-1 Dlb^-1 Ul^H = R^-1 Q^H (Dlb^-1 Ul^H); log|det [.]| = sum log |R_kk|
       ProfScope ps(g, DQMC_K_INVERSE);
-      LCHK(launch_qr_blocked(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D5), 0, g.wg_qr));
+      LCHK(launch_qr_blocked(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D5), 0, g.wg_qr, g.qr_nb));
       LCHK(launch_adjoint(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_UL)));
       LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_TMP2), NV(V_D1)));
-      LCHK(launch_qr_solve(s(), d(), nc(), g.n, NM(M_TMP), NM(M_TMP2)));
+      LCHK(launch_qr_solve(s(), d(), nc(), g.n, NM(M_TMP), NM(M_TMP2), g.qr_nb));
       LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_TMP2), NV(V_D3)));
       LCHK(launch_sum_log(s(), d(), nc(), g.n, NV(V_D5), S_LOGDET_A, 0));
     }
@@ -1857,6 +1862,216 @@ int dqmc_profile(dqmc_ctx* ctx, unsigned class_mask) {
 int dqmc_profile_read(dqmc_ctx* ctx, double* ms, long long* count, int nclass) {
   CTX_CHECK(ctx);
   return dqmc_batch_profile_read(ctx->b, ms, count, nclass);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: per shape group, time the local-update kernel over (variant, workgroups per
+// chain) and the blocked QR over (workgroups per chain, panel width) on the group's live
+// state, keep the fastest. The chains' Green's function, field, scalars and uniform cursor
+// are saved before and restored after every timed launch (the local updates run on
+// synthetic uniforms), so tuning leaves the Monte Carlo state bit for bit as it was.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TuneSave {
+  cplx* G = nullptr;
+  double* phi = nullptr;
+  double* scal = nullptr;
+  long long* cur = nullptr;
+  int* err = nullptr;
+  double* uni = nullptr;
+};
+
+int autotune_group(dqmc_batch* b, Group& g, double box, int reps, double* rep) {
+  const int nch = g.nch();
+  hipStream_t st = g.stream;
+  std::vector<void*> tmp;
+  auto talloc = [&](void** p, size_t bytes) -> int {
+    if (hipMalloc(p, bytes) != hipSuccess) return fail(DQMC_E_HIP, "autotune: hipMalloc");
+    tmp.push_back(*p);
+    return 0;
+  };
+  auto release = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (void* p : tmp) (void)hipFree(p);
+    tmp.clear();
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = 0;
+  std::vector<TuneSave> sv(nch);
+  std::vector<ChainDesc> keep(nch);
+  // synthetic uniforms for one slice of proposals (host MT19937, fixed seed)
+  std::mt19937_64 eng(20240611);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  for (int k = 0; k < nch && !rc; ++k) {
+    ChainMem& cm = b->chains[g.chains[k]];
+    const size_t nn = (size_t)cm.n * cm.n, nf = (size_t)3 * cm.N * cm.M;
+    if ((rc = talloc((void**)&sv[k].G, nn * sizeof(cplx)))) break;
+    if ((rc = talloc((void**)&sv[k].phi, nf * sizeof(double)))) break;
+    if ((rc = talloc((void**)&sv[k].scal, kNumScalar * sizeof(double)))) break;
+    if ((rc = talloc((void**)&sv[k].cur, sizeof(long long)))) break;
+    if ((rc = talloc((void**)&sv[k].err, sizeof(int)))) break;
+    if ((rc = talloc((void**)&sv[k].uni, (size_t)4 * cm.N * sizeof(double)))) break;
+    std::vector<double> hu((size_t)4 * cm.N);
+    for (auto& x : hu) x = U(eng);
+    HIPCHK(hipMemcpy(sv[k].uni, hu.data(), hu.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(sv[k].G, cm.d.mat[M_G], nn * sizeof(cplx), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sv[k].phi, cm.d.phi, nf * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sv[k].scal, cm.d.scal, kNumScalar * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sv[k].cur, cm.d.ucursor, sizeof(long long), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sv[k].err, cm.d.error, sizeof(int), hipMemcpyDeviceToDevice, st));
+    keep[k] = cm.d;
+  }
+  if (rc) {
+    release();
+    return rc;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  auto restore = [&]() -> int {
+    for (int k = 0; k < nch; ++k) {
+      ChainMem& cm = b->chains[g.chains[k]];
+      const size_t nn = (size_t)cm.n * cm.n, nf = (size_t)3 * cm.N * cm.M;
+      HIPCHK(hipMemcpyAsync(cm.d.mat[M_G], sv[k].G, nn * sizeof(cplx), hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(cm.d.phi, sv[k].phi, nf * sizeof(double), hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(cm.d.scal, sv[k].scal, kNumScalar * sizeof(double), hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(cm.d.ucursor, sv[k].cur, sizeof(long long), hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpyAsync(cm.d.error, sv[k].err, sizeof(int), hipMemcpyDeviceToDevice, st));
+    }
+    return 0;
+  };
+  // the local updates read the synthetic uniforms from a cursor at 0
+  long long zero = 0;
+  for (int k = 0; k < nch; ++k) {
+    ChainMem& cm = b->chains[g.chains[k]];
+    cm.d.uniforms = sv[k].uni;
+    cm.d.n_uniforms = 4 * cm.N;
+  }
+  if ((rc = upload_desc(b, g))) {
+    release();
+    return rc;
+  }
+  auto timed = [&](auto&& launch, auto&& reset, float* best) -> hipError_t {
+    *best = 1e30f;
+    for (int r = 0; r <= reps; ++r) {  // the first launch is a warm-up
+      if (reset()) return hipErrorUnknown;
+      hipError_t e = hipEventRecord(e0, st);
+      if (e == hipSuccess) e = launch();
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(st);
+        return e;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (r > 0) *best = std::min(*best, ms);
+    }
+    return hipSuccess;
+  };
+  const int cap = std::max(1, std::min({kMaxWG, std::max(1, 1024 / nch), g.n / 2}));
+  std::vector<int> wgs;
+  for (int w : {1, 2, 3, 4, 6, 8, 12, 16, 25, 32, 50, 64, 100, 128})
+    if (w <= cap) wgs.push_back(w);
+  // ---- local updates
+  auto reset_local = [&]() -> int {
+    if (restore()) return 1;
+    for (int k = 0; k < nch; ++k)
+      if (hipMemcpyAsync(b->chains[g.chains[k]].d.ucursor, &zero, sizeof(long long), hipMemcpyHostToDevice, st) !=
+          hipSuccess)
+        return 1;
+    return 0;
+  };
+  float best_local = 1e30f;
+  int best_w = g.wg_local, best_v = g.local_variant;
+  for (int w : wgs)
+    for (int v : {4, 2, 3}) {
+      if (v == 4 && local_window_sites(g.n, w) < 2) continue;
+      if (v == 2 && !local_tile_fits(g.n, w)) continue;
+      if (v == 3 && !local_delayed_fits(g.n)) continue;
+      float ms = 0.f;
+      const hipError_t e = timed([&]() { return launch_local_updates(st, g.d_desc, nch, g.n, g.cur - 1, nullptr, box,
+                                                                     w, 0, v); },
+                                 reset_local, &ms);
+      if (e == hipSuccess && ms < best_local) {
+        best_local = ms;
+        best_w = w;
+        best_v = v;
+      }
+    }
+  // ---- blocked QR of a copy of G (the stack's factorisation: pre-pivoted)
+  auto reset_qr = [&]() -> int {
+    for (int k = 0; k < nch; ++k) {
+      ChainMem& cm = b->chains[g.chains[k]];
+      if (hipMemcpyAsync(cm.d.mat[M_TMP], sv[k].G, (size_t)cm.n * cm.n * sizeof(cplx), hipMemcpyDeviceToDevice, st) !=
+          hipSuccess)
+        return 1;
+    }
+    return 0;
+  };
+  float best_qr = 1e30f;
+  int best_wq = g.wg_qr, best_nb = g.qr_nb;
+  for (int nb : {16, 8}) {
+    if (nb == 16 && g.n > 512) continue;
+    for (int w : wgs) {
+      if (w > g.n / nb) continue;
+      float ms = 0.f;
+      const hipError_t e = timed([&]() { return launch_qr_blocked(st, g.d_desc, nch, g.n, MatRef{MK_NAMED, M_TMP},
+                                                                  VecRef{VK_NAMED, V_D5}, 1, w, nb); },
+                                 reset_qr, &ms);
+      if (e == hipSuccess && ms < best_qr) {
+        best_qr = ms;
+        best_wq = w;
+        best_nb = nb;
+      }
+    }
+  }
+  // ---- restore the chains and apply
+  for (int k = 0; k < nch; ++k) b->chains[g.chains[k]].d = keep[k];
+  rc = restore();
+  if (!rc) rc = upload_desc(b, g);
+  release();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  g.drop_graphs();
+  if (best_local < 1e29f) {
+    g.wg_local = best_w;
+    g.local_variant = best_v;
+  }
+  if (best_qr < 1e29f) {
+    g.wg_qr = best_wq;
+    g.qr_nb = best_nb;
+  }
+  if (rep) {
+    rep[0] = g.wg_local;
+    rep[1] = g.local_variant;
+    rep[2] = best_local < 1e29f ? best_local : -1.0;
+    rep[3] = g.wg_qr;
+    rep[4] = g.qr_nb;
+    rep[5] = best_qr < 1e29f ? best_qr : -1.0;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int dqmc_batch_autotune(dqmc_batch* b, const double* box, int reps, double* report) {
+  if (!b || !box) return fail(DQMC_E_ARG, "bad autotune args");
+  RC(batch_sync(b));
+  for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+    Group& g = b->groups[gi];
+    if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "autotune needs an initialised batch (dqmc_batch_init)");
+    RC(autotune_group(b, g, box[g.chains[0]], std::max(1, reps), report ? report + 6 * gi : nullptr));
+  }
+  return 0;
+}
+
+int dqmc_autotune(dqmc_ctx* ctx, double box, int reps, double* report) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_autotune(ctx->b, &box, reps, report);
 }
 
 int dqmc_batch_use_graph(dqmc_batch* b, int enable) {

@@ -524,10 +524,12 @@ hipError_t launch_qr_blocked_nb(hipStream_t s, const ChainDesc* chains, int ncha
 }
 
 hipError_t launch_qr_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
-                             int prepivot, int wg) {
+                             int prepivot, int wg, int nb) {
   if (nmax > kQrMaxN) return hipErrorInvalidValue;
-  return qr_panel_width(nmax) == 16 ? launch_qr_blocked_nb<16>(s, chains, nchains, nmax, A, D, prepivot, wg)
-                                    : launch_qr_blocked_nb<8>(s, chains, nchains, nmax, A, D, prepivot, wg);
+  if (nb == 0) nb = qr_panel_width(nmax);
+  if (nb == 16 && nmax <= 512) return launch_qr_blocked_nb<16>(s, chains, nchains, nmax, A, D, prepivot, wg);
+  if (nb == 8) return launch_qr_blocked_nb<8>(s, chains, nchains, nmax, A, D, prepivot, wg);
+  return hipErrorInvalidValue;
 }
 
 template <int NB, int CB>
@@ -544,9 +546,11 @@ hipError_t launch_qr_solve_nb(hipStream_t s, const ChainDesc* chains, int nchain
   return hipGetLastError();
 }
 
-hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B) {
+hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B, int nb) {
   if (nmax > kQrMaxN || nmax < 16) return hipErrorInvalidValue;
-  if (qr_panel_width(nmax) == 16) return launch_qr_solve_nb<16, 4>(s, chains, nchains, nmax, A, B);
+  if (nb == 0) nb = qr_panel_width(nmax);
+  if (nb == 16 && nmax <= 512) return launch_qr_solve_nb<16, 4>(s, chains, nchains, nmax, A, B);
+  if (nb != 8) return hipErrorInvalidValue;
   if (nmax <= 640) return launch_qr_solve_nb<8, 4>(s, chains, nchains, nmax, A, B);
   return launch_qr_solve_nb<8, 1>(s, chains, nchains, nmax, A, B);
 }

@@ -213,6 +213,15 @@ class DQMC:
         replay it; same results as eager launches."""
         check(self.lib.dqmc_use_graph(self.h, 1 if enable else 0))
 
+    def autotune(self, reps=3, box=None):
+        """Time the local updates over (variant, workgroups) and the blocked QR over
+        (workgroups, panel width) on the current state and keep the fastest; the state is
+        restored exactly. Returns the choice as a dict."""
+        box = self.p.box if box is None else box
+        rep = np.zeros(6)
+        check(self.lib.dqmc_autotune(self.h, box, reps, dptr(rep)))
+        return _tune_report(rep)
+
     def set_greens_route(self, route):
         """0: D-scale separation (default); 1: the reference's calculate_greens verbatim."""
         check(self.lib.dqmc_set_greens_route(self.h, route))
@@ -497,6 +506,19 @@ class Batch:
         """Capture each shape group's sweep in a HIP graph and replay it (see DQMC.use_graph)."""
         check(self.lib.dqmc_batch_use_graph(self.h, 1 if enable else 0))
 
+    def autotune(self, reps=3):
+        """DQMC.autotune for every shape group; returns one dict per group."""
+        box = np.array([p.box for p in self.params])
+        rep = np.zeros(6 * len(self.params))
+        check(self.lib.dqmc_batch_autotune(self.h, dptr(box), reps, dptr(rep)))
+        # shape groups, in order of first appearance (dqmc_batch_create)
+        keys = []
+        for p in self.params:
+            k = (p.L, p.slices, p.safe_mult, bool(p.checkerboard), bool(p.all_checks))
+            if k not in keys:
+                keys.append(k)
+        return [_tune_report(rep[6 * g:6 * g + 6]) for g in range(len(keys))]
+
     def greens(self, chain):
         n = self.params[chain].n
         G = np.zeros((n, n), dtype=np.complex128, order="F")
@@ -507,6 +529,11 @@ class Batch:
         st = _capi.Stats()
         check(self.lib.dqmc_batch_get_stats(self.h, chain, ctypes.byref(st)))
         return st
+
+
+def _tune_report(r):
+    return {"local_workgroups": int(r[0]), "local_variant": int(r[1]), "local_ms": float(r[2]),
+            "qr_workgroups": int(r[3]), "qr_panel": int(r[4]), "qr_ms": float(r[5])}
 
 
 def _hmc_trajectory(self, momenta, nsteps, eps, u):

@@ -254,6 +254,16 @@ int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st);
  * results as eager launches; not used while kernel timing (dqmc_batch_profile) is on. */
 int dqmc_batch_use_graph(dqmc_batch* b, int enable);
 int dqmc_use_graph(dqmc_ctx* ctx, int enable);
+/* Autotuner. No reference counterpart (the reference has one fixed CPU code path). Per
+ * shape group: times the local-update kernel over (variant, workgroups per chain) and the
+ * blocked QR of the stack over (workgroups per chain, panel width 16 or 8) on the group's
+ * current state, `reps` timed launches each after a warm-up, and keeps the fastest. The
+ * chains' G, field, scalars and uniform cursor are restored bit for bit (local updates are
+ * timed on synthetic uniforms), so tuning does not change the Monte Carlo trajectory. Needs
+ * an initialised batch. report (optional): 6 doubles per group -- workgroups and variant of
+ * the local updates and their time in ms, workgroups and panel width of the QR and its time. */
+int dqmc_batch_autotune(dqmc_batch* b, const double* box, int reps, double* report);
+int dqmc_autotune(dqmc_ctx* ctx, double box, int reps, double* report);
 /* dqmc_ctx_set_groups / dqmc_set_local_variant for every shape group of a batch */
 int dqmc_batch_set_groups(dqmc_batch* b, int wg_qr, int wg_inv, int wg_local);
 int dqmc_batch_set_local_variant(dqmc_batch* b, int variant);
