@@ -102,6 +102,8 @@ SIGNATURES = {
     "dqmc_batch_get_stats": [_P, _i, ctypes.POINTER(Stats)],
     "dqmc_batch_use_graph": [_P, _i],
     "dqmc_use_graph": [_P, _i],
+    "dqmc_set_qr_panel": [_P, _i],
+    "dqmc_batch_set_qr_panel": [_P, _i],
     "dqmc_batch_autotune": [_P, _D, _i, _D],
     "dqmc_autotune": [_P, ctypes.c_double, _i, _D],
     "dqmc_batch_set_groups": [_P, _i, _i, _i],

@@ -196,6 +196,11 @@ hipError_t launch_udt(hipStream_t s, const ChainDesc* chains, int nchains, int n
                       MatRef Qout, MatRef Tout, int wg);
 // the two halves of launch_udt: the pivoted QR itself, then Q and T formation
 hipError_t launch_qrp(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D, int wg);
+hipError_t launch_form_T(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
+                         MatRef T);
+// Q of a factorisation by launch_qr_blocked (same nb), from its compact-WY panels
+hipError_t launch_form_Q_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef Q,
+                                 int nb = 0);
 hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
                           MatRef Qout, MatRef Tout);
 

@@ -88,6 +88,22 @@ __device__ inline cplx wave_sum(cplx v) {
   return v;
 }
 
+// XCD-aware workgroup numbering. The dispatcher hands consecutive workgroups of a launch
+// to the 8 XCDs in turn, and each XCD has its own L2. Renumbering so that every XCD gets a
+// contiguous range of logical blocks puts the workgroups of one chain (blockIdx.y) on as
+// few XCDs as possible, where they share the chain's data in L2. A pure relabelling of a
+// (gridDim.x, gridDim.y) grid: every logical block is run exactly once.
+constexpr int kNumXcd = 8;
+__device__ inline void xcd_block(int& bx, int& by) {
+  const int X = gridDim.x, total = X * gridDim.y;
+  const int pid = blockIdx.y * X + blockIdx.x;
+  const int per = total / kNumXcd, rem = total % kNumXcd;
+  const int k = pid % kNumXcd, slot = pid / kNumXcd;
+  const int l = k * per + min(k, rem) + slot;
+  by = l / X;
+  bx = l - by * X;
+}
+
 // Block-wide sum; `red` is LDS scratch of >= blockDim.x/64 doubles (x2 for cplx).
 __device__ inline double block_sum(double v, double* red) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;

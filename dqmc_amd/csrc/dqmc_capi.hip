@@ -401,7 +401,12 @@ struct Ops {
         LCHK(launch_qrp(s(), d(), nc(), g.n, A, D, g.wg_qr));
     }
     ProfScope ps(g, DQMC_K_QFORM);
-    LCHK(launch_form_TQ(s(), d(), nc(), g.n, A, D, Q, T));
+    if (stack && g.udt_route == 0) {
+      LCHK(launch_form_T(s(), d(), nc(), g.n, A, D, T));
+      LCHK(launch_form_Q_blocked(s(), d(), nc(), g.n, A, Q, g.qr_nb));
+    } else {
+      LCHK(launch_form_TQ(s(), d(), nc(), g.n, A, D, Q, T));
+    }
     return 0;
   }
   int inverse(MatRef X) {
@@ -1198,6 +1203,24 @@ int dqmc_batch_set_local_variant(dqmc_batch* b, int variant) {
     g.local_variant = variant;
   }
   return 0;
+}
+
+int dqmc_batch_set_qr_panel(dqmc_batch* b, int nb) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  if (nb != 0 && nb != 8 && nb != 16) return fail(DQMC_E_ARG, "QR panel width must be 0, 8 or 16");
+  for (auto& g : b->groups)
+    if (nb == 16 && g.n > 512) return fail(DQMC_E_ARG, "16-column QR panels need n <= 512");
+  RC(batch_sync(b));
+  for (auto& g : b->groups) {
+    g.drop_graphs();
+    g.qr_nb = nb ? nb : (g.nch() <= 8 ? 8 : 0);  // the default of default_groups
+  }
+  return 0;
+}
+
+int dqmc_set_qr_panel(dqmc_ctx* ctx, int nb) {
+  CTX_CHECK(ctx);
+  return dqmc_batch_set_qr_panel(ctx->b, nb);
 }
 
 int dqmc_set_local_variant(dqmc_ctx* ctx, int variant) {

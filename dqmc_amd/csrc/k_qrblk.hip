@@ -367,9 +367,11 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
 template <int NB, int CB>
 __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRef Ar, MatRef Br) {
   constexpr int RS = 4 / CB;
-  const ChainDesc& c = chains[blockIdx.y];
+  int bx, by;
+  xcd_block(bx, by);
+  const ChainDesc& c = chains[by];
   const int n = c.n;
-  const int j0 = blockIdx.x * CB;
+  const int j0 = bx * CB;
   if (j0 >= n) return;
   const int ncol = min(CB, n - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -504,6 +506,107 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
   }
 }
 
+// Q = Q_1 Q_2 ... Q_P of a factorisation by k_qr_blocked, Q_p = I - V_p T_p V_p^H
+// (zungqr's result, formed from the compact-WY panels instead of one reflector at a time).
+// Each workgroup owns CB columns of Q in LDS, starting from the identity, and applies the
+// panels from the last one that reaches its columns down to the first; the waves split as
+// in k_qr_solve. Reflector k lives in physical column qr_perm[k] below its diagonal.
+template <int NB, int CB>
+__global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains, MatRef Ar, MatRef Qr) {
+  constexpr int RS = 4 / CB;
+  int bx, by;
+  xcd_block(bx, by);
+  const ChainDesc& c = chains[by];
+  const int n = c.n;
+  const int j0 = bx * CB;
+  if (j0 >= n) return;
+  const int ncol = min(CB, n - j0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int jl = wave % CB, rs = wave / CB;
+  const cplx* A = matptr(c, Ar);
+  cplx* Q = matptr(c, Qr);
+  extern __shared__ double smem[];
+  cplx* xc = reinterpret_cast<cplx*>(smem);  // [CB][n]
+  cplx* V = xc + (size_t)CB * n;              // [NB][n]
+  __shared__ cplx Tl[NB * NB];
+  __shared__ cplx part[RS > 1 ? 4 * NB : 1];
+  for (int r = tid; r < n; r += 256)
+#pragma unroll
+    for (int q = 0; q < CB; ++q) {
+      xc[(size_t)q * n + r].re = (r == j0 + q) ? 1.0 : 0.0;
+      xc[(size_t)q * n + r].im = 0.0;
+    }
+  const bool active = jl < ncol;  // wave-uniform
+  cplx* a = xc + (size_t)jl * n;
+  const int jmax = j0 + ncol - 1;  // panels starting beyond it leave these columns alone
+  for (int p = jmax / NB; p >= 0; --p) {
+    const int k0 = p * NB, kb = min(NB, n - k0);
+    __syncthreads();
+    stage_panel<NB>(V, A, c.qr_perm, k0, kb, n, tid, 1);  // columns >= kb zero-filled
+    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = c.qr_Tf[(size_t)p * NB * NB + e];
+    __syncthreads();
+    cplx wv[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) wv[j] = mk(0, 0);
+    for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
+      const cplx ak = a[k];
+      cplx pj[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], ak);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) wv[j] = wsum(wv[j]);
+    if (RS > 1) {
+      if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) part[wave * NB + j] = wv[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        cplx sj = mk(0, 0);
+#pragma unroll
+        for (int q = 0; q < RS; ++q) sj += part[(jl + CB * q) * NB + j];
+        wv[j] = sj;
+      }
+    }
+    // z = T w: lane j < NB forms z_j, then every lane reads all of them
+    const int jz = lane & (NB - 1);
+    cplx zl = mk(0, 0);
+#pragma unroll
+    for (int l = 0; l < NB; ++l) cfma(zl, Tl[jz * NB + l], wv[l]);
+    cplx z[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) z[j] = read_lane_c(zl, j);
+    if (active) {
+      for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
+        cplx ak = a[k];
+        cplx pj[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) ak = ak - pj[j] * z[j];
+        a[k] = ak;
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += 256) {
+    cplx v[CB];
+#pragma unroll
+    for (int q = 0; q < CB; ++q) v[q] = xc[(size_t)q * n + r];
+#pragma unroll
+    for (int q = 0; q < CB; ++q)
+      if (q < ncol) {
+        cplx* d = Q + (size_t)(j0 + q) * n + r;
+        d->re = v[q].re;
+        d->im = v[q].im;
+      }
+  }
+}
+
 int qr_panel_width(int nmax) { return nmax <= 512 ? 16 : 8; }
 
 template <int NB>
@@ -544,6 +647,32 @@ hipError_t launch_qr_solve_nb(hipStream_t s, const ChainDesc* chains, int nchain
   }
   hipLaunchKernelGGL((k_qr_solve<NB, CB>), dim3((nmax + CB - 1) / CB, nchains), dim3(256), lds, s, chains, A, B);
   return hipGetLastError();
+}
+
+template <int NB, int CB>
+hipError_t launch_form_Q_blocked_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A,
+                                    MatRef Q) {
+  const size_t lds = (size_t)(CB + NB) * nmax * sizeof(cplx);
+  static size_t attr = 0;
+  if (lds > 65536 && lds > attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_form_Q_blocked<NB, CB>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    attr = lds;
+  }
+  hipLaunchKernelGGL((k_form_Q_blocked<NB, CB>), dim3((nmax + CB - 1) / CB, nchains), dim3(256), lds, s, chains, A,
+                     Q);
+  return hipGetLastError();
+}
+
+hipError_t launch_form_Q_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef Q,
+                                 int nb) {
+  if (nmax > kQrMaxN || nmax < 16) return hipErrorInvalidValue;
+  if (nb == 0) nb = qr_panel_width(nmax);
+  if (nb == 16 && nmax <= 512) return launch_form_Q_blocked_nb<16, 4>(s, chains, nchains, nmax, A, Q);
+  if (nb != 8) return hipErrorInvalidValue;
+  if (nmax <= 640) return launch_form_Q_blocked_nb<8, 4>(s, chains, nchains, nmax, A, Q);
+  return launch_form_Q_blocked_nb<8, 1>(s, chains, nchains, nmax, A, Q);
 }
 
 hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B, int nb) {

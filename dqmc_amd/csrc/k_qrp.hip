@@ -323,13 +323,20 @@ hipError_t launch_udt(hipStream_t s, const ChainDesc* chains, int nchains, int n
   return launch_form_TQ(s, chains, nchains, nmax, A, D, Q, T);
 }
 
-hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
-                          MatRef Q, MatRef T) {
+hipError_t launch_form_T(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
+                         MatRef T) {
   if (nmax > kMaxN) return hipErrorInvalidValue;
   const size_t nn = (size_t)nmax * nmax;
   int blocks = (int)((nn + 255) / 256);
   if (blocks > 1024) blocks = 1024;
   hipLaunchKernelGGL(k_form_T, dim3(blocks, nchains), dim3(256), 0, s, chains, A, D, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
+                          MatRef Q, MatRef T) {
+  hipError_t e0 = launch_form_T(s, chains, nchains, nmax, A, D, T);
+  if (e0 != hipSuccess) return e0;
   const int cb = nmax <= 512 ? 8 : 4;
   const size_t lds = (size_t)2 * cb * nmax * sizeof(cplx);
   static size_t attr = 0;

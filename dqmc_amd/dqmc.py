@@ -208,6 +208,10 @@ class DQMC:
         4 windowed delayed updates."""
         check(self.lib.dqmc_set_local_variant(self.h, variant))
 
+    def set_qr_panel(self, nb):
+        """Panel width of the blocked QR: 16 (n <= 512) or 8; 0 restores the default."""
+        check(self.lib.dqmc_set_qr_panel(self.h, nb))
+
     def use_graph(self, enable=True):
         """Capture each sweep's kernel sequence in a HIP graph (one per starting state) and
         replay it; same results as eager launches."""

@@ -100,14 +100,19 @@ def test_udt_matches_lapack(amd):
     c.close()
 
 
-def test_udt_prepivoted_blocked_qr(amd):
+@pytest.mark.parametrize("panel", [8, 16])
+def test_udt_prepivoted_blocked_qr(amd, panel):
     """The stack's factorisation: U unitary, U diag(D) T = A column by column, T upper
-    triangular after the pre-pivoting order, D descending for a column-graded matrix."""
+    triangular after the pre-pivoting order, D descending for a column-graded matrix; both
+    panel widths of the blocked QR (16 only up to n = 512)."""
     mod, DQMC = amd
     rng = np.random.default_rng(7)
     for L in (4, 10, 16):
-        _say("blocked pre-pivoted QR n=%d" % (4 * L * L))
+        if panel == 16 and 4 * L * L > 512:
+            continue
+        _say("blocked pre-pivoted QR n=%d, %d-column panels" % (4 * L * L, panel))
         c = DQMC(config_params(mod, L, 10))
+        c.set_qr_panel(panel)
         n = c.p.n
         scales = 10.0 ** rng.uniform(-30, 30, n)
         A = rand_c(rng, n) * scales[None, :]
@@ -189,12 +194,19 @@ def test_slice_matrix_config_size(amd):
 
 @pytest.mark.parametrize("route", [0, 1])
 def test_calculate_greens_kat(amd, route):
-    # tests_O3.jl:215-230; route 0 = D-scale separation, 1 = the reference's two-UDT route
+    # tests_O3.jl:215-230; route 0 = D-scale separation (blocked QR with 8- and with
+    # 16-column panels), 1 = the reference's two-UDT route
     mod, DQMC = amd
-    _say("calculate_greens KAT route %d" % route)
+    for panel in ((8, 16) if route == 0 else (8,)):
+        _say("calculate_greens KAT route %d, %d-column panels" % (route, panel))
+        _greens_kat_case(mod, DQMC, route, panel)
+
+
+def _greens_kat_case(mod, DQMC, route, panel):
     p = fixture_params(mod)
     c = DQMC(p)
     c.set_greens_route(route)
+    c.set_qr_panel(panel)
     conf = reconstruct_seeded_conf(ld)
     c.hsfield = conf
     G, ldet = c.calculate_greens(ld("Ul"), ld("Dl"), ld("Tl"), ld("Ur"), ld("Dr"), ld("Tr"))
@@ -208,15 +220,16 @@ def test_calculate_greens_kat(amd, route):
 
 
 # --------------------------------------------------------------------------- chain
-@pytest.mark.parametrize("route", [0, 1])
-def test_propagation_full_cycle_fixture(amd, route):
+@pytest.mark.parametrize("route,panel", [(0, 8), (0, 16), (1, 8)])
+def test_propagation_full_cycle_fixture(amd, route, panel):
     # tests_O3.jl:240-263: every slice of a full up-down cycle within 1e-12 of the direct G
     mod, DQMC = amd
-    _say("propagation up-down cycle L=4 M=10 (bfield fixture model), route %d" % route)
+    _say("propagation up-down cycle L=4 M=10 (bfield fixture model), route %d, %d-column panels" % (route, panel))
     p = fixture_params(mod)
     conf = reconstruct_seeded_conf(ld)
     c = DQMC(p)
     c.set_greens_route(route)
+    c.set_qr_panel(panel)
     c.init(conf)
     assert (c.current_slice, c.direction) == (10, -1)
     o = O.DQMC(oracle_params(p), conf)
