@@ -866,26 +866,68 @@ __global__ __launch_bounds__(256) void k_local_updates_delayed(const ChainDesc* 
 
 // ---------------------------------------------------------------------------
 // Variant 4: windowed delayed updates. The sites of the slice are split into R windows
-// (R even, at most K sites each). Within a window every workgroup keeps the accepted
-// updates as factor pairs in LDS -- A_b = (G'[:, c] - E) M^-1 (n x 4, all rows) and
-// B_b = delta G'[r, :] restricted to its own columns -- together with the current 4x4
-// blocks G'[r_j, c_k] between the window's sites, so decisions and new factor pairs
-// need no communication. At the end of a window each workgroup writes its columns of
-// G + sum_b A_b B_b into the partner buffer (M_G and M_GALT alternate; R is even, so the
-// slice ends in M_G), then one group barrier: one barrier per window instead of one per
-// accepted proposal. Global G is read only from the current window's buffer, which no
-// workgroup writes during the window. With TILE the workgroup's own columns stay in
-// LDS for the whole slice, so the end-of-window pass reads nothing from global memory.
+// (R even, at most K sites each). Inside a window no workgroup communicates:
+//  * the accepted updates are kept implicitly. With G0 the window's starting G and
+//    Yt = [G0[:, c_b] - E_b]_b the starting site columns of the accepted sites (minus their
+//    unit columns), every factor A_b = (G'[:, c_b] - E_b) M_b^-1 equals Yt P_b for a small
+//    coefficient matrix P_b (4K x 4): P_b = (S_b + sum_{b'<b} P_b' B_b'[:, c_b]) M_b^-1, S_b
+//    selecting block b. Only P_b, the 4x4 blocks A_b(r_j, :) and B_b(:, c_j) at the window's
+//    sites, B_b on the workgroup's own columns, and the current blocks G'(r_j, c_k) between
+//    the window's sites are stored (LDS), so a window's size is not bounded by n x 4 factors;
+//  * decisions read G'(r_j, c_j); new factor pairs need only the stored blocks and G0 on
+//    the own columns.
+// At the window's end each workgroup writes its columns of G0 + Yt W, W = sum_b P_b B_b
+// (formed in place of the B_b), into the partner buffer (M_G and M_GALT alternate; R is
+// even, so the slice ends in M_G), then one group barrier. Global G is read only from the
+// current window's buffer, which no workgroup writes during the window. With TILE the own
+// columns also stay in LDS for the slice.
 // Same arithmetic as update_greens! (local_updates.jl:61-95) up to the order of sums.
 // ---------------------------------------------------------------------------
-constexpr int kWinMax = 8;
+constexpr int kWinMax = 12;     // window sites with LDS-resident own columns
+constexpr int kWinMaxGlob = 6;  // window sites otherwise (the end-of-window rows keep 4K in registers)
 constexpr size_t kWindowLdsMax = 160 * 1024 - 1024;  // the CU's LDS less this kernel's static part
 
 __host__ __device__ inline int window_bo_stride(int n, int wg) { return (n + wg - 1) / wg + 1; }
 __host__ __device__ inline size_t window_lds_bytes(int n, int N, int wg, int K, bool tile) {
   const size_t ncs = window_bo_stride(n, wg);
   const size_t t = tile ? (size_t)(ncs - 1) * n : 0;
-  return slice_lds_doubles(N) * 8 + (t + (size_t)K * n * 4 + (size_t)K * 4 * ncs + (size_t)2 * K * K * 16) * 16;
+  return slice_lds_doubles(N) * 8 + (t + (size_t)8 * K * ncs + (size_t)48 * K * K + (size_t)32 * K) * 16;
+}
+
+// End-of-window rows without LDS-resident columns: Gn[:, own] = Gc[:, own] + Yt W, with the
+// row of Yt (4 MB entries, zero past 4m) in registers and 4 own columns per step; m >= 1.
+template <int MB>
+__device__ inline void window_rows_glob(const cplx* __restrict__ Gc, cplx* __restrict__ Gn, const cplx* W,
+                                        const int* accs, int m, int n, int N, int cb0, int ncol, int ncs) {
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    cplx y[4 * MB];
+#pragma unroll
+    for (int t = 0; t < 4 * MB; ++t) {
+      const int b = min(t >> 2, m - 1), k = t & 3;
+      const int row = accs[b] + k * N;
+      cplx v = Gc[(size_t)row * n + r];
+      if (r == row) v.re -= 1.0;
+      y[t] = (t < 4 * m) ? v : mk(0, 0);
+    }
+    for (int j0 = 0; j0 < ncol; j0 += 4) {
+      cplx acc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = Gc[(size_t)(cb0 + min(j0 + q, ncol - 1)) * n + r];
+#pragma unroll
+      for (int t = 0; t < 4 * MB; ++t) {
+        const cplx* w = W + (size_t)min(t, 4 * m - 1) * ncs;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cfma(acc[q], y[t], w[min(j0 + q, ncol - 1)]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (j0 + q < ncol) {
+          cplx* d = Gn + (size_t)(cb0 + j0 + q) * n + r;
+          d->re = acc[q].re;
+          d->im = acc[q].im;
+        }
+    }
+  }
 }
 
 template <bool TILE>
@@ -899,15 +941,20 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
   const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
   const int ncol = cb1 - cb0;
   const int ncs = window_bo_stride(n, W);
+  const int K4 = 4 * K;
 
   extern __shared__ double smem[];
   const SliceLds s = load_slice(c, l, smem);
   cplx* T = reinterpret_cast<cplx*>(smem + slice_lds_doubles(N));  // [ncs - 1][n] own columns (TILE)
-  cplx* Aw = T + (TILE ? (size_t)(ncs - 1) * n : 0);                 // [K][4][n]  A_b (rows contiguous)
-  cplx* Bo = Aw + (size_t)K * n * 4;                                 // [K][4][ncs] B_b, own columns
-  cplx* Gp = Bo + (size_t)K * 4 * ncs;                               // [K][K][16] G'[r_j, c_k]
-  cplx* Bw = Gp + (size_t)K * K * 16;                                // [K][K][16] B_b[:, c_k]
+  cplx* Bo = T + (TILE ? (size_t)(ncs - 1) * n : 0);  // [K][4][ncs] B_b on own columns
+  cplx* Hs = Bo + (size_t)K4 * ncs;  // [K][4][ncs] H_j = sum_b A_b(r_j, :) B_b(:, own); W at a window's end
+  cplx* Gp = Hs + (size_t)K4 * ncs;  // [K][K][16]  G'(r_j, c_k)
+  cplx* Pc = Gp + (size_t)16 * K * K;  // [K][4K][4] P_b
+  cplx* Us = Pc + (size_t)16 * K * K;  // [K][4K][4] U_k = sum_b P_b B_b(:, c_k)
+  cplx* Bw = Us + (size_t)16 * K * K;  // [K][16]    B_m(:, c_k) of the update being applied
+  cplx* Ar = Bw + (size_t)16 * K;      // [K][16]    A_m(r_j, :)
   __shared__ LocalShared sh;
+  __shared__ int accs[kWinMax];  // sites accepted in the current window, in order
   cplx* X[2] = {c.mat[M_G], c.mat[M_GALT]};
   int R = (N + K - 1) / K;
   R += R & 1;
@@ -933,11 +980,13 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
     const int s0 = (int)((long long)rw * N / R), s1 = (int)((long long)(rw + 1) * N / R), kw = s1 - s0;
     const cplx* Gc = X[rw & 1];
     cplx* Gn = X[(rw + 1) & 1];
-    // Gp[j][k](a, b) = G(r_j + a N, c_k + b N) for the window's sites j, k
+    // Gp[j][k](a, b) = G(r_j + a N, c_k + b N) for the window's sites j, k; U = 0, H = 0
     for (int e = tid; e < kw * kw * 16; e += blockDim.x) {
       const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
       Gp[(jj * K + kk) * 16 + a * 4 + b] = Gc[(size_t)(s0 + kk + b * N) * n + (s0 + jj + a * N)];
     }
+    for (int e = tid; e < kw * K4 * 4; e += blockDim.x) Us[e] = mk(0, 0);
+    for (int e = tid; e < kw * 4 * ncs; e += blockDim.x) Hs[e] = mk(0, 0);
     __syncthreads();
     int m = 0;
     while (!stop) {
@@ -957,62 +1006,13 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
       stop = sh.stop;
       if (site >= s1) break;
       const int ii = site - s0;
-      cplx* Am = Aw + (size_t)m * n * 4;
-      // A = (G'[:, c_i] - E) M^-1 with G'[:, c_i] = G[:, c_i] + sum_b A_b B_b[:, c_i];
-      // two rows per thread, their eight loads issued together
-      for (int r0 = tid; r0 < n; r0 += 2 * blockDim.x) {
-        const int r1 = min(r0 + (int)blockDim.x, n - 1);
-        cplx g4[2][4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          g4[0][k] = Gc[(size_t)(site + k * N) * n + r0];
-          g4[1][k] = Gc[(size_t)(site + k * N) * n + r1];
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int r = h ? r1 : r0;
-          for (int b = 0; b < m; ++b) {
-            const cplx* ab = Aw + (size_t)b * 4 * n + r;
-            const cplx* bw = Bw + (b * K + ii) * 16;
-            const cplx a0 = ab[0], a1 = ab[n], a2 = ab[2 * n], a3 = ab[3 * n];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              cfma(g4[h][k], a0, bw[k]);
-              cfma(g4[h][k], a1, bw[4 + k]);
-              cfma(g4[h][k], a2, bw[8 + k]);
-              cfma(g4[h][k], a3, bw[12 + k]);
-            }
-          }
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (r == site + k * N) g4[h][k].re -= 1.0;
-          if (h == 0 || r0 + (int)blockDim.x < n) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              cplx acc = mk(0, 0);
-#pragma unroll
-              for (int q = 0; q < 4; ++q) cfma(acc, g4[h][q], sh.minv[q * 4 + k]);
-              Am[(size_t)k * n + r] = acc;
-            }
-          }
-        }
-      }
-      // B = delta G'[r_i, own columns] with G'[r_i, :] = G[r_i, :] + sum_b A_b[r_i, :] B_b
+      // B_m on own columns: delta (G0(r_i, .) + H_i)
       for (int jl = tid; jl < ncol; jl += blockDim.x) {
         cplx g4[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 4; ++q) {
           g4[q] = TILE ? T[(size_t)jl * n + site + q * N] : Gc[(size_t)(cb0 + jl) * n + site + q * N];
-        for (int b = 0; b < m; ++b) {
-          cplx bo[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) bo[k] = Bo[((size_t)b * 4 + k) * ncs + jl];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const cplx* ab = Aw + (size_t)b * 4 * n + site + q * N;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) cfma(g4[q], ab[(size_t)k * n], bo[k]);
-          }
+          g4[q] += Hs[((size_t)ii * 4 + q) * ncs + jl];
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -1022,60 +1022,120 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
           Bo[((size_t)m * 4 + k) * ncs + jl] = acc;
         }
       }
-      // B[:, c_k] for the window's sites: delta G'[r_i, c_k]
+      // B_m(:, c_k) = delta G'(r_i, c_k) and A_m(r_j, :) = G'(r_j, c_i) M^-1 for the sites
+      // ahead (j, k > i: the only ones read again)
       for (int e = tid; e < kw * 16; e += blockDim.x) {
         const int kk = e >> 4, k = (e >> 2) & 3, b = e & 3;
+        if (kk <= ii) continue;
+        cplx bw = mk(0, 0), ar = mk(0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          cfma(bw, sh.delta[k * 4 + q], Gp[(ii * K + kk) * 16 + q * 4 + b]);
+          cfma(ar, Gp[(kk * K + ii) * 16 + k * 4 + q], sh.minv[q * 4 + b]);
+        }
+        Bw[kk * 16 + k * 4 + b] = bw;
+        Ar[kk * 16 + k * 4 + b] = ar;
+      }
+      // P_m = (S_m + U_i) M^-1 (rows of blocks <= m)
+      for (int e = tid; e < 16 * (m + 1); e += blockDim.x) {
+        const int t = e >> 2, k = e & 3;
         cplx acc = mk(0, 0);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) cfma(acc, sh.delta[k * 4 + q], Gp[(ii * K + kk) * 16 + q * 4 + b]);
-        Bw[(m * K + kk) * 16 + k * 4 + b] = acc;
+        for (int q = 0; q < 4; ++q) {
+          cplx v = Us[((size_t)ii * K4 + t) * 4 + q];
+          if (t == 4 * m + q) v.re += 1.0;
+          cfma(acc, v, sh.minv[q * 4 + k]);
+        }
+        Pc[((size_t)m * K4 + t) * 4 + k] = acc;
       }
+      if (tid == 0) accs[m] = site;
       __syncthreads();
-      // the blocks of the sites still ahead: G'[r_j, c_k] += A[r_j, :] B[:, c_k]
-      for (int e = tid; e < kw * kw * 16; e += blockDim.x) {
-        const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
-        if (jj <= ii) continue;
-        const cplx* ar = Am + s0 + jj + a * N;
+      // the sums kept for the sites ahead (j, k > i):
+      //   G'(r_j, c_k) += A_m(r_j, :) B_m(:, c_k),  U_k += P_m B_m(:, c_k),  H_j += A_m(r_j, :) B_m(:, own)
+      const int ka = kw - ii - 1;
+      for (int e = tid; e < ka * ka * 16; e += blockDim.x) {
+        const int jk = e >> 4, jj = ii + 1 + jk / ka, kk = ii + 1 + jk % ka, a = (e >> 2) & 3, b = e & 3;
         cplx acc = Gp[(jj * K + kk) * 16 + a * 4 + b];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) cfma(acc, ar[(size_t)q * n], Bw[(m * K + kk) * 16 + q * 4 + b]);
+        for (int q = 0; q < 4; ++q) cfma(acc, Ar[jj * 16 + a * 4 + q], Bw[kk * 16 + q * 4 + b]);
         Gp[(jj * K + kk) * 16 + a * 4 + b] = acc;
+      }
+      const int nu = 16 * (m + 1);
+      for (int e = tid; e < ka * nu; e += blockDim.x) {
+        const int kk = ii + 1 + e / nu, r2 = e % nu, t = r2 >> 2, q = r2 & 3;
+        cplx acc = Us[((size_t)kk * K4 + t) * 4 + q];
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) cfma(acc, Pc[((size_t)m * K4 + t) * 4 + q2], Bw[kk * 16 + q2 * 4 + q]);
+        Us[((size_t)kk * K4 + t) * 4 + q] = acc;
+      }
+      const int nh = 4 * ncol;
+      for (int e = tid; e < ka * nh; e += blockDim.x) {
+        const int jj = ii + 1 + e / nh, r2 = e % nh, q = r2 / ncol, jl = r2 - q * ncol;
+        cplx acc = Hs[((size_t)jj * 4 + q) * ncs + jl];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cfma(acc, Ar[jj * 16 + q * 4 + k], Bo[((size_t)m * 4 + k) * ncs + jl]);
+        Hs[((size_t)jj * 4 + q) * ncs + jl] = acc;
       }
       ++m;
       __syncthreads();
     }
-    // own columns of the partner buffer: G + sum_b A_b B_b (a plain copy when m = 0)
-    for (int r = tid; r < n; r += blockDim.x) {
-      for (int j0 = 0; j0 < ncol; j0 += 4) {
-        cplx acc[4];
+    // W = sum_b P_b B_b (into the H storage, no longer needed): entry (t, jl) sums over the b
+    // with P_b nonzero in row t (b >= t / 4)
+    cplx* Wb = Hs;
+    for (int e = tid; e < 4 * m * ncol; e += blockDim.x) {
+      const int t = e / ncol, jl = e - t * ncol;
+      cplx acc = mk(0, 0);
+      for (int b = t >> 2; b < m; ++b) {
+        const cplx* pb = Pc + ((size_t)b * K4 + t) * 4;
+        const cplx* bo = Bo + (size_t)b * 4 * ncs + jl;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int jl = min(j0 + t, ncol - 1);
-          acc[t] = TILE ? T[(size_t)jl * n + r] : Gc[(size_t)(cb0 + jl) * n + r];
-        }
-        for (int b = 0; b < m; ++b) {
-          const cplx* ab = Aw + (size_t)b * 4 * n + r;
-          const cplx a0 = ab[0], a1 = ab[n], a2 = ab[2 * n], a3 = ab[3 * n];
+        for (int q = 0; q < 4; ++q) cfma(acc, pb[q], bo[(size_t)q * ncs]);
+      }
+      Wb[(size_t)t * ncs + jl] = acc;
+    }
+    __syncthreads();
+    // own columns of the partner buffer: G0 + Yt W (a plain copy when m = 0)
+    if (TILE) {
+      for (int r = tid; r < n; r += blockDim.x)
+        for (int j0 = 0; j0 < ncol; j0 += 4) {
+          cplx acc[4];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const int jl = min(j0 + t, ncol - 1);
-            cfma(acc[t], a0, Bo[((size_t)b * 4 + 0) * ncs + jl]);
-            cfma(acc[t], a1, Bo[((size_t)b * 4 + 1) * ncs + jl]);
-            cfma(acc[t], a2, Bo[((size_t)b * 4 + 2) * ncs + jl]);
-            cfma(acc[t], a3, Bo[((size_t)b * 4 + 3) * ncs + jl]);
-          }
-        }
+          for (int q = 0; q < 4; ++q) acc[q] = T[(size_t)min(j0 + q, ncol - 1) * n + r];
+          for (int b = 0; b < m; ++b) {
+            cplx y[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-          if (j0 + t < ncol) {
-            if (TILE) {
-              T[(size_t)(j0 + t) * n + r].re = acc[t].re;
-              T[(size_t)(j0 + t) * n + r].im = acc[t].im;
+            for (int k = 0; k < 4; ++k) {
+              const int row = accs[b] + k * N;
+              y[k] = Gc[(size_t)row * n + r];
+              if (r == row) y[k].re -= 1.0;
             }
-            cplx* d = Gn + (size_t)(cb0 + j0 + t) * n + r;
-            d->re = acc[t].re;
-            d->im = acc[t].im;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const cplx* wr = Wb + ((size_t)b * 4 + k) * ncs;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) cfma(acc[q], y[k], wr[min(j0 + q, ncol - 1)]);
+            }
           }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (j0 + q < ncol) {
+              T[(size_t)(j0 + q) * n + r].re = acc[q].re;
+              T[(size_t)(j0 + q) * n + r].im = acc[q].im;
+              cplx* d = Gn + (size_t)(cb0 + j0 + q) * n + r;
+              d->re = acc[q].re;
+              d->im = acc[q].im;
+            }
+        }
+    } else if (m == 0) {
+      for (int e = tid; e < ncol * n; e += blockDim.x) {
+        const cplx v = Gc[(size_t)cb0 * n + e];
+        Gn[(size_t)cb0 * n + e] = v;
+      }
+    } else {
+      switch ((m + 1) >> 1) {
+        case 1: window_rows_glob<2>(Gc, Gn, Wb, accs, m, n, N, cb0, ncol, ncs); break;
+        case 2: window_rows_glob<4>(Gc, Gn, Wb, accs, m, n, N, cb0, ncol, ncs); break;
+        default: window_rows_glob<6>(Gc, Gn, Wb, accs, m, n, N, cb0, ncol, ncs); break;
       }
     }
     if (rw + 1 < R) {
@@ -1087,15 +1147,15 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
 }
 
 // the window of variant 4 at this shape: its K, and whether the own columns fit in LDS too
-// (preferred when the window keeps at least 3 sites); K = 0 when nothing fits
+// (preferred when the window keeps at least 4 sites); K = 0 when nothing fits
 int local_window_sites(int nmax, int wg, bool* tile) {
-  for (int K = kWinMax; K >= 3; --K)
+  for (int K = kWinMax; K >= 4; --K)
     if (window_lds_bytes(nmax, nmax / 4, wg, K, true) <= kWindowLdsMax) {
       if (tile) *tile = true;
       return K;
     }
   if (tile) *tile = false;
-  for (int K = kWinMax; K >= 2; --K)
+  for (int K = kWinMaxGlob; K >= 2; --K)
     if (window_lds_bytes(nmax, nmax / 4, wg, K, false) <= kWindowLdsMax) return K;
   return 0;
 }
