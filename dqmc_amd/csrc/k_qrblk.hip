@@ -6,7 +6,7 @@
 // column-graded products DQMC stabilises, (B...B U) diag(D), QR with pre-pivoting is
 // as stable as column-pivoted QR (Bai, Lee, Li, Xu, "Stable solutions of linear systems
 // involving long chain of matrix multiplications", Linear Algebra Appl. 435 (2011)),
-// while needing only two group barriers per panel instead of one per column.
+// while needing one group barrier per panel instead of one per column.
 //
 // The output uses the layout of k_qrp (columns never move: logical column k lives in
 // physical column qr_perm[k]; reflector k below the diagonal of that column, R on and
@@ -14,9 +14,10 @@
 // decompose_udt (linalg.jl:20-39) unchanged. The T factors of all panels are kept in
 // qr_Tf for k_qr_solve.
 //
-// Per panel: workgroup 0 factors the panel in LDS (zlarfg per column, applied to the
-// rest of the panel, then zlarft), writes it back, group barrier; every workgroup
-// applies Q_p^H to its own trailing columns (one wave per column), group barrier.
+// Per panel p, with one-panel look-ahead: workgroup 0 applies Q_p^H to panel p+1's columns
+// and factors panel p+1 in LDS (zlarfg per column, applied to the rest of the panel, then
+// zlarft) while the other workgroups apply Q_p^H to the columns beyond (one wave per
+// column); one group barrier per panel.
 #include "chain.hpp"
 
 namespace dqmc {
@@ -128,224 +129,238 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   }
   __syncthreads();
 
-  for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
+  // ---- the panel factorisation of panel p (workgroup 0 alone, in LDS)
+  auto factor_panel = [&](int k0, int p) {
     const int kb = min(NB, n - k0);
     cplx* Tg = c.qr_Tf + (size_t)p * NB * NB;
-    if (w == 0) {
-      // ---- panel factorisation in LDS
-      stage_panel<NB>(P, A, perm, k0, kb, n, tid, 0);
-      __syncthreads();
-      // per column: wave 0 forms the reflector (zlarfg) alone, one barrier publishes it, every
-      // wave applies it to its share of the remaining panel columns, one barrier
-      for (int t = 0; t < kb; ++t) {
-        const int i = k0 + t;
-        cplx* x = P + (size_t)t * n;
-        if (wave == 0) {
-          double part = 0.0;
+    stage_panel<NB>(P, A, perm, k0, kb, n, tid, 0);
+    __syncthreads();
+    // per column: wave 0 forms the reflector (zlarfg) alone, one barrier publishes it, every
+    // wave applies it to its share of the remaining panel columns, one barrier
+    for (int t = 0; t < kb; ++t) {
+      const int i = k0 + t;
+      cplx* x = P + (size_t)t * n;
+      if (wave == 0) {
+        double part = 0.0;
+        for (int kq = i + 1 + lane; kq < n; kq += 256) {
+          cplx v[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v[q] = x[min(kq + 64 * q, n - 1)];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) part += (kq + 64 * q < n) ? abs2(v[q]) : 0.0;
+        }
+        const double xnorm = sqrt(wave_sum(part));
+        const cplx alpha = x[i];
+        cplx tau;
+        double beta;
+        // zlarfg: H^H (alpha, x) = (beta, 0), v = (1, x / (alpha - beta))
+        if (xnorm == 0.0 && alpha.im == 0.0) {
+          tau = mk(0, 0);
+          beta = alpha.re;
+        } else {
+          beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm * xnorm), alpha.re);
+          tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+          const cplx sc = cinv(alpha - mk(beta, 0));
           for (int kq = i + 1 + lane; kq < n; kq += 256) {
             cplx v[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[q] = x[min(kq + 64 * q, n - 1)];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) part += (kq + 64 * q < n) ? abs2(v[q]) : 0.0;
-          }
-          const double xnorm = sqrt(wave_sum(part));
-          const cplx alpha = x[i];
-          cplx tau;
-          double beta;
-          // zlarfg: H^H (alpha, x) = (beta, 0), v = (1, x / (alpha - beta))
-          if (xnorm == 0.0 && alpha.im == 0.0) {
-            tau = mk(0, 0);
-            beta = alpha.re;
-          } else {
-            beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm * xnorm), alpha.re);
-            tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
-            const cplx sc = cinv(alpha - mk(beta, 0));
-            for (int kq = i + 1 + lane; kq < n; kq += 256) {
-              cplx v[4];
-#pragma unroll
-              for (int q = 0; q < 4; ++q) v[q] = x[min(kq + 64 * q, n - 1)];
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                const int k = kq + 64 * q;
-                *(k < n ? x + k : sink + tid) = v[q] * sc;
-              }
-            }
-          }
-          if (lane == 0) {  // global copies are written after the panel: a store here
-            taul[t] = tau;   // would make every later __syncthreads wait for it
-            betal[t] = beta;
-          }
-        }
-        __syncthreads();
-        // H^H = I - conj(tau) v v^H on the remaining panel columns: wave w takes columns
-        // c0 = t+1+w, c0+4, ... (nu of them, wave-uniform); their loads and sums overlap
-        const int c0 = t + 1 + wave;
-        const int nu = c0 < kb ? (kb - c0 + 3) >> 2 : 0;
-        if (nu > 0) {
-          const cplx ctau = conj(taul[t]);
-          cplx* col[KC];
-#pragma unroll
-          for (int u = 0; u < KC; ++u) col[u] = (u < nu) ? P + (size_t)(c0 + 4 * u) * n : x;
-          cplx acc[KC];
-#pragma unroll
-          for (int u = 0; u < KC; ++u) acc[u] = mk(0, 0);
-          for (int kq = i + lane; kq < n; kq += 128) {
-            cplx vk[2], pv[2][KC];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-              const int kk = min(kq + 64 * q, n - 1);
-              vk[q] = x[kk];
-#pragma unroll
-              for (int u = 0; u < KC; ++u) pv[q][u] = col[u][kk];
-            }
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < 4; ++q) {
               const int k = kq + 64 * q;
-              const cplx v = (k == i) ? mk(1, 0) : (k < n) ? vk[q] : mk(0, 0);
-#pragma unroll
-              for (int u = 0; u < KC; ++u) cfma_cj(acc[u], v, pv[q][u]);
-            }
-          }
-          cplx f[KC];
-#pragma unroll
-          for (int u = 0; u < KC; ++u) f[u] = (u < nu) ? ctau * wsum(acc[u]) : mk(0, 0);
-          for (int kq = i + lane; kq < n; kq += 128) {
-            cplx vk[2], pv[2][KC];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-              const int kk = min(kq + 64 * q, n - 1);
-              vk[q] = x[kk];
-#pragma unroll
-              for (int u = 0; u < KC; ++u) pv[q][u] = col[u][kk];
-            }
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-              const int k = kq + 64 * q;
-              const cplx v = (k == i) ? mk(1, 0) : vk[q];
-#pragma unroll
-              for (int u = 0; u < KC; ++u) *((u < nu && k < n) ? col[u] + k : sink + tid) = pv[q][u] - f[u] * v;
+              *(k < n ? x + k : sink + tid) = v[q] * sc;
             }
           }
         }
-        __syncthreads();
-      }
-      // R's diagonal goes where the reflectors' implicit ones were (never read again)
-      if (tid < kb) {
-        P[(size_t)tid * n + k0 + tid] = mk(betal[tid], 0);
-        D[k0 + tid] = fabs(betal[tid]);
-        c.qr_tau[k0 + tid] = taul[tid];
-      }
-      // ---- zlarft (forward, columnwise): T(t,t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) V(:, 0:t)^H v_t
-      // first the inner products S(j, t) = V(:, j)^H v_t, j < t, into Tl(j, t); wave w takes
-      // t = 1+w, 5+w, ...; the products of one t share v_t's loads. Columns j >= t are read
-      // too (in bounds) and their sums dropped.
-      for (int t = 1 + wave; t < kb; t += nwaves) {
-        const cplx* vt = P + (size_t)t * n;
-        const int it = k0 + t;
-        cplx acc[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) acc[j] = mk(0, 0);
-        for (int k = it + lane; k < n; k += 64) {
-          const cplx b = (k == it) ? mk(1, 0) : vt[k];
-          cplx pj[NB];
-#pragma unroll
-          for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + k];  // rows >= it: below v_j's diagonal
-#pragma unroll
-          for (int j = 0; j < NB; ++j) cfma_cj(acc[j], pj[j], b);
+        if (lane == 0) {  // global copies are written after the panel: a store here
+          taul[t] = tau;   // would make every later __syncthreads wait for it
+          betal[t] = beta;
         }
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-          if (j < t) {
-            const cplx sj = wsum(acc[j]);
-            if (lane == 0) Tl[j * NB + t] = sj;
-          }
       }
       __syncthreads();
-      // the triangle, by wave 0: lane j keeps row j of T in registers; column t needs
-      // sum_{l<t} T(j, l) S(l, t), where T(j, l) = 0 for j > l
-      if (wave == 0) {
-        cplx trow[NB];
+      // H^H = I - conj(tau) v v^H on the remaining panel columns: wave w takes columns
+      // c0 = t+1+w, c0+4, ... (nu of them, wave-uniform); their loads and sums overlap
+      const int c0 = t + 1 + wave;
+      const int nu = c0 < kb ? (kb - c0 + 3) >> 2 : 0;
+      if (nu > 0) {
+        const cplx ctau = conj(taul[t]);
+        cplx* col[KC];
 #pragma unroll
-        for (int l = 0; l < NB; ++l) trow[l] = mk(0, 0);
+        for (int u = 0; u < KC; ++u) col[u] = (u < nu) ? P + (size_t)(c0 + 4 * u) * n : x;
+        cplx acc[KC];
+#pragma unroll
+        for (int u = 0; u < KC; ++u) acc[u] = mk(0, 0);
+        for (int kq = i + lane; kq < n; kq += 128) {
+          cplx vk[2], pv[2][KC];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int kk = min(kq + 64 * q, n - 1);
+            vk[q] = x[kk];
+#pragma unroll
+            for (int u = 0; u < KC; ++u) pv[q][u] = col[u][kk];
+          }
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int k = kq + 64 * q;
+            const cplx v = (k == i) ? mk(1, 0) : (k < n) ? vk[q] : mk(0, 0);
+#pragma unroll
+            for (int u = 0; u < KC; ++u) cfma_cj(acc[u], v, pv[q][u]);
+          }
+        }
+        cplx f[KC];
+#pragma unroll
+        for (int u = 0; u < KC; ++u) f[u] = (u < nu) ? ctau * wsum(acc[u]) : mk(0, 0);
+        for (int kq = i + lane; kq < n; kq += 128) {
+          cplx vk[2], pv[2][KC];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int kk = min(kq + 64 * q, n - 1);
+            vk[q] = x[kk];
+#pragma unroll
+            for (int u = 0; u < KC; ++u) pv[q][u] = col[u][kk];
+          }
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int k = kq + 64 * q;
+            const cplx v = (k == i) ? mk(1, 0) : vk[q];
+#pragma unroll
+            for (int u = 0; u < KC; ++u) *((u < nu && k < n) ? col[u] + k : sink + tid) = pv[q][u] - f[u] * v;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // R's diagonal goes where the reflectors' implicit ones were (never read again)
+    if (tid < kb) {
+      P[(size_t)tid * n + k0 + tid] = mk(betal[tid], 0);
+      D[k0 + tid] = fabs(betal[tid]);
+      c.qr_tau[k0 + tid] = taul[tid];
+    }
+    // ---- zlarft (forward, columnwise): T(t,t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) V(:, 0:t)^H v_t
+    // first the inner products S(j, t) = V(:, j)^H v_t, j < t, into Tl(j, t); wave w takes
+    // t = 1+w, 5+w, ...; the products of one t share v_t's loads. Columns j >= t are read
+    // too (in bounds) and their sums dropped.
+    for (int t = 1 + wave; t < kb; t += nwaves) {
+      const cplx* vt = P + (size_t)t * n;
+      const int it = k0 + t;
+      cplx acc[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) acc[j] = mk(0, 0);
+      for (int k = it + lane; k < n; k += 64) {
+        const cplx b = (k == it) ? mk(1, 0) : vt[k];
+        cplx pj[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + k];  // rows >= it: below v_j's diagonal
+#pragma unroll
+        for (int j = 0; j < NB; ++j) cfma_cj(acc[j], pj[j], b);
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        if (j < t) {
+          const cplx sj = wsum(acc[j]);
+          if (lane == 0) Tl[j * NB + t] = sj;
+        }
+    }
+    __syncthreads();
+    // the triangle, by wave 0: lane j keeps row j of T in registers; column t needs
+    // sum_{l<t} T(j, l) S(l, t), where T(j, l) = 0 for j > l
+    if (wave == 0) {
+      cplx trow[NB];
+#pragma unroll
+      for (int l = 0; l < NB; ++l) trow[l] = mk(0, 0);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        if (t < kb) {
+          cplx acc = mk(0, 0);
+#pragma unroll
+          for (int l = 0; l < t; ++l) cfma(acc, trow[l], Tl[l * NB + t]);
+          const cplx tt = taul[t];
+          trow[t] = (lane < t) ? -(tt * acc) : (lane == t) ? tt : mk(0, 0);
+        }
+      }
+      if (lane < NB) {
 #pragma unroll
         for (int t = 0; t < NB; ++t) {
-          if (t < kb) {
-            cplx acc = mk(0, 0);
-#pragma unroll
-            for (int l = 0; l < t; ++l) cfma(acc, trow[l], Tl[l * NB + t]);
-            const cplx tt = taul[t];
-            trow[t] = (lane < t) ? -(tt * acc) : (lane == t) ? tt : mk(0, 0);
-          }
-        }
-        if (lane < NB) {
-#pragma unroll
-          for (int t = 0; t < NB; ++t) {
-            Tl[lane * NB + t] = trow[t];
-            Tg[lane * NB + t] = trow[t];
-          }
+          Tl[lane * NB + t] = trow[t];
+          Tg[lane * NB + t] = trow[t];
         }
       }
-      __syncthreads();
-      unstage_panel<NB>(A, P, perm, k0, kb, n, tid);
     }
-    group_sync(g);
-    if (group_failed(g)) return;
-    if (k0 + kb >= n) break;
-    // ---- trailing update: a <- (I - V T^H V^H) a for the own columns of logical position >= k0 + NB
-    // (only the last panel can be narrower than NB, and it has no trailing columns)
-    stage_panel<NB>(P, A, perm, k0, kb, n, tid, 1);
-    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = Tg[e];
     __syncthreads();
-    for (int col = w + W * wave; col < n; col += W * nwaves) {
-      if (lpos[col] < k0 + NB) continue;
-      cplx* a = A + (size_t)col * n;
-      // the column's rows k0.. in registers (all loads in flight together)
-      cplx av[QN];
+    unstage_panel<NB>(A, P, perm, k0, kb, n, tid);
+  };
+  // ---- a <- (I - V T^H V^H) a for one column, with panel V (from row k0) staged in P and T in Tl
+  auto apply_col = [&](int col, int k0) {
+    cplx* a = A + (size_t)col * n;
+    // the column's rows k0.. in registers (all loads in flight together)
+    cplx av[QN];
 #pragma unroll
-      for (int q = 0; q < QN; ++q) {
-        const int k = k0 + lane + 64 * q;
-        av[q] = k < n ? a[k] : mk(0, 0);
+    for (int q = 0; q < QN; ++q) {
+      const int k = k0 + lane + 64 * q;
+      av[q] = k < n ? a[k] : mk(0, 0);
+    }
+    cplx wv[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) wv[j] = mk(0, 0);
+#pragma unroll
+    for (int q = 0; q < QN; ++q) {
+      const int k = k0 + lane + 64 * q;
+      if (k0 + 64 * q < n) {  // uniform
+        const int kk = min(k, n - 1);
+        cplx pj[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + kk];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], av[q]);  // av = 0 past row n-1
       }
-      cplx wv[NB];
+    }
 #pragma unroll
-      for (int j = 0; j < NB; ++j) wv[j] = mk(0, 0);
+    for (int j = 0; j < NB; ++j) wv[j] = wsum(wv[j]);
+    // z = T^H w: lane j < NB forms z_j, then every lane reads all NB of them
+    const int jl = lane & (NB - 1);
+    cplx zl = mk(0, 0);
 #pragma unroll
-      for (int q = 0; q < QN; ++q) {
-        const int k = k0 + lane + 64 * q;
-        if (k0 + 64 * q < n) {  // uniform
-          const int kk = min(k, n - 1);
-          cplx pj[NB];
+    for (int l = 0; l < NB; ++l) cfma_cj(zl, Tl[l * NB + jl], wv[l]);
+    cplx z[NB];
 #pragma unroll
-          for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + kk];
+    for (int j = 0; j < NB; ++j) z[j] = read_lane_c(zl, j);
 #pragma unroll
-          for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], av[q]);  // av = 0 past row n-1
-        }
+    for (int q = 0; q < QN; ++q) {
+      const int k = k0 + lane + 64 * q;
+      if (k0 + 64 * q < n) {
+        const int kk = min(k, n - 1);
+        cplx pj[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + kk];
+        cplx ak = av[q];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) ak = ak - pj[j] * z[j];
+        if (k < n) a[k] = ak;
       }
-#pragma unroll
-      for (int j = 0; j < NB; ++j) wv[j] = wsum(wv[j]);
-      // z = T^H w: lane j < NB forms z_j, then every lane reads all NB of them
-      const int jl = lane & (NB - 1);
-      cplx zl = mk(0, 0);
-#pragma unroll
-      for (int l = 0; l < NB; ++l) cfma_cj(zl, Tl[l * NB + jl], wv[l]);
-      cplx z[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) z[j] = read_lane_c(zl, j);
-#pragma unroll
-      for (int q = 0; q < QN; ++q) {
-        const int k = k0 + lane + 64 * q;
-        if (k0 + 64 * q < n) {
-          const int kk = min(k, n - 1);
-          cplx pj[NB];
-#pragma unroll
-          for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + kk];
-          cplx ak = av[q];
-#pragma unroll
-          for (int j = 0; j < NB; ++j) ak = ak - pj[j] * z[j];
-          if (k < n) a[k] = ak;
-        }
-      }
+    }
+  };
+
+  // One-panel look-ahead: while every other workgroup applies panel p to the columns beyond
+  // panel p+1, workgroup 0 applies it to panel p+1's columns and factors panel p+1, so each
+  // panel costs one group barrier and its factorisation overlaps the trailing update.
+  if (w == 0) factor_panel(0, 0);
+  group_sync(g);
+  if (group_failed(g)) return;
+  for (int k0 = 0, p = 0; k0 + NB < n; k0 += NB, ++p) {  // panel p (full width) is factored
+    const int k1 = k0 + NB, kb1 = min(NB, n - k1);
+    stage_panel<NB>(P, A, perm, k0, NB, n, tid, 1);
+    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = c.qr_Tf[(size_t)p * NB * NB + e];
+    __syncthreads();
+    if (w == 0) {
+      for (int u = wave; u < kb1; u += nwaves) apply_col(perm[k1 + u], k0);
+      if (W == 1)  // nobody else to take the trailing columns
+        for (int col = wave; col < n; col += nwaves)
+          if (lpos[col] >= k1 + NB) apply_col(col, k0);
+      __syncthreads();
+      factor_panel(k1, p + 1);
+    } else {
+      for (int col = (w - 1) + (W - 1) * wave; col < n; col += (W - 1) * nwaves)
+        if (lpos[col] >= k1 + NB) apply_col(col, k0);
     }
     group_sync(g);
     if (group_failed(g)) return;
