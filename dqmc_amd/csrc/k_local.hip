@@ -250,9 +250,10 @@ struct Proposals {
   int nacc, nonreal, i;
 };
 
-// Evaluate proposals from site p.i on, in wave 0 (every lane holds entry (ea, eb) of the 4x4
-// algebra; lanes 16..63 mirror 0..15), until one is accepted: its delta and M^-1 go to
-// sh.delta / sh.minv, the field of the site is updated in LDS and the site is returned.
+// Evaluate proposals from site p.i on, in wave 0 (each 16-lane row of the wave holds one
+// site's 4x4 algebra, lane (ea, eb) of the row one entry), until one is accepted: its delta
+// and M^-1 go to sh.delta / sh.minv, the field of the site is updated in LDS and the site is
+// returned.
 // Sites from p.i up to limit - 1 are visited; returns limit when none of them is accepted
 // (p.i = limit) or when the uniform stream ran out (*stop set, p.i = the site it stopped at).
 // diag(j) returns this lane's entry G_jj(ea, eb) of the current G.
@@ -262,6 +263,7 @@ template <class Diag>
 __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposals& p, LocalShared& sh,
                                   double box, int dolog, bool logger, Diag diag, int* stop, int limit) {
   const int N = c.N, lane = threadIdx.x & 63;
+  const int row = lane >> 4;  // 16-lane row of the wave: it evaluates site i + row
   const int ea = (lane & 15) >> 2, eb = lane & 3;
   int t1[4], t2[4];  // entry types of row ea (old operator) and column (ea + eb) & 3 (new operator)
 #pragma unroll
@@ -274,20 +276,23 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
   const double k_r = 0.5 * c.r, k_u = 0.25 * c.u;
   const int edrun = c.edrun;
   *stop = 0;
+  // The four rows evaluate four consecutive sites at once. Row r takes site i + r with the
+  // uniforms it gets if sites i .. i+r-1 are all rejected -- a rejection needs p_acc <= 1 and
+  // so always draws 4 -- from the current G and field, which those rejections leave as they
+  // are. The rows are then resolved in order, exactly as one proposal after another; the
+  // first acceptance ends the call and the later rows' work is dropped.
   int i = p.i;
-  cplx gnext = mk(0, 0);  // G_ii of the next site, loaded one site ahead
-  if (i < limit) gnext = diag(i);
-  for (; i < limit; ++i) {
+  cplx gnext = mk(0, 0);  // this row's G_ii, loaded one step ahead
+  if (i < limit) gnext = diag(min(i + row, limit - 1));
+  for (; i < limit; i += 4) {
+    const int si = min(i + row, limit - 1);
     const cplx gv = gnext;
-    if (i + 1 < limit) gnext = diag(i + 1);
-    if ((p.cur - s.cur0) + 4 > s.navail) {
-      *stop = 1;
-      break;
-    }
-    const double* u = s.ul + (p.cur - s.cur0);
+    if (i + 4 < limit) gnext = diag(min(i + 4 + row, limit - 1));
+    const long long off = min((p.cur - s.cur0) + 4 * row, max(s.navail - 4, 0ll));  // in bounds
+    const double* u = s.ul + off;
     double old[3], nw[3];
     for (int a = 0; a < 3; ++a) {
-      old[a] = s.phl[3 * i + a];
+      old[a] = s.phl[3 * si + a];
       nw[a] = old[a] + (-box + 2.0 * box * u[a]);
     }
     const double old_sq = old[0] * old[0] + old[1] * old[1] + old[2] * old[2];
@@ -297,8 +302,8 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
     double dt_dot = 0.0, ds_dot = 0.0;
     for (int a = 0; a < 3; ++a) {
       const double diff = nw[a] - old[a];
-      dt_dot += s.tnb[3 * i + a] * diff;
-      ds_dot += s.snb[3 * i + a] * diff;
+      dt_dot += s.tnb[3 * si + a] * diff;
+      ds_dot += s.snb[3 * si + a] * diff;
     }
     double dS;
     if (!edrun) {
@@ -313,10 +318,11 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
     double C2, R2;
     cplx S2;
     ev_coeffs_op(lam_dtau, nw[0], nw[1], nw[2], 1.0, C2, S2, R2);
-    const double C1 = s.e1c[4 * i], R1 = s.e1c[4 * i + 3];
-    const cplx S1 = mk(s.e1c[4 * i + 1], s.e1c[4 * i + 2]);
+    const double C1 = s.e1c[4 * si], R1 = s.e1c[4 * si + 3];
+    const cplx S1 = mk(s.e1c[4 * si + 1], s.e1c[4 * si + 2]);
     // this lane's entry of delta is (ea, (ea + eb) & 3): the skew lets the product below
-    // take both operands from DPP moves (quad broadcasts of delta, row rotations of I - G)
+    // take both operands from DPP moves (quad broadcasts of delta, row rotations of I - G);
+    // every DPP move stays inside the 16-lane row
     cplx dm = mk(0, 0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) cfma(dm, ev_entry_t(C1, S1, R1, t1[k]), ev_entry_t(C2, S2, R2, t2[k]));
@@ -329,7 +335,7 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
     cfma(mm, dpp_c<0xAA>(dm), dpp_c<0x128>(gm));
     cfma(mm, dpp_c<0x55>(dm), dpp_c<0x12C>(gm));
     if (ea == eb) mm.re += 1.0;
-    cplx Mall[16];  // every lane: all of M, by row broadcasts of each lane of the 16-lane row
+    cplx Mall[16];  // every lane: all of its row's M, by row broadcasts of each lane of the row
     Mall[0] = dpp_c<0x150>(mm);
     Mall[1] = dpp_c<0x151>(mm);
     Mall[2] = dpp_c<0x152>(mm);
@@ -347,57 +353,74 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
     Mall[14] = dpp_c<0x15E>(mm);
     Mall[15] = dpp_c<0x15F>(mm);
     const cplx det = det4_laplace(Mall);
-    if (fabs(det.im) > 1e-4 * fabs(det.re)) ++p.nonreal;
-    const double p_acc = exp_dS * det.re;
-    int acc;
-    double uval = -1.0;
-    if (p_acc > 1.0) {
-      acc = 1;
-      p.cur += 3;
-    } else {
-      uval = u[3];
-      acc = uval < p_acc;
-      p.cur += 4;
-      p.minratio = fmin(p.minratio, fabs(p_acc - uval));
-    }
-    if (dolog && logger && lane == 0 && p.logi < c.log_cap) {
-      double* lg = c.prop_log + 4 * p.logi;
-      lg[0] = p_acc;
-      lg[1] = uval;
-      lg[2] = acc;
-      lg[3] = det.im / det.re;
-    }
-    ++p.logi;
-    if (acc) {
-      ++p.nacc;
-      p.dS_acc += -log(exp_dS);
-      if (lane == 0)
-        for (int a = 0; a < 3; ++a) s.phl[3 * i + a] = nw[a];
-      // the four neighbours' space sums, recomputed in the original order
-      if (lane < 12) {
-        const int j = s.nbl[(lane / 3) * N + i], a = lane - 3 * (lane / 3);
-        double sn = 0.0;
-        for (int k = 0; k < 4; ++k) sn += s.phl[3 * s.nbl[k * N + j] + a];
-        s.snb[3 * j + a] = sn;
+    const double p_row = exp_dS * det.re;
+    const double u3 = u[3];
+    // resolve the rows in order (wave-uniform)
+    for (int g = 0; g < 4; ++g) {
+      const int site = i + g;
+      if (site >= limit) break;
+      if ((p.cur - s.cur0) + 4 > s.navail) {
+        *stop = 1;
+        p.i = site;
+        return limit;
       }
-      // M^-1 (b, a) = cofactor (a, b) / det M, one entry per lane
-      cplx m3[9];
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) m3[r * 3 + q] = shfl_c(mm, (r + (r >= ea)) * 4 + q + (q >= eb));
-      cplx cof = m3[0] * (m3[4] * m3[8] - m3[5] * m3[7]) - m3[1] * (m3[3] * m3[8] - m3[5] * m3[6]) +
-                 m3[2] * (m3[3] * m3[7] - m3[4] * m3[6]);
-      if ((ea + eb) & 1) cof = -cof;
-      if (lane < 16) {
-        sh.delta[ea * 4 + ((ea + eb) & 3)] = dm;
-        sh.minv[eb * 4 + ea] = cof * cinv(det);
+      const double p_acc = read_lane(p_row, 16 * g);
+      const double dre = read_lane(det.re, 16 * g), dim = read_lane(det.im, 16 * g);
+      if (fabs(dim) > 1e-4 * fabs(dre)) ++p.nonreal;
+      int acc;
+      double uval = -1.0;
+      if (p_acc > 1.0) {
+        acc = 1;
+        p.cur += 3;
+      } else {
+        uval = read_lane(u3, 16 * g);
+        acc = uval < p_acc;
+        p.cur += 4;
+        p.minratio = fmin(p.minratio, fabs(p_acc - uval));
       }
-      p.i = i + 1;
-      return i;
+      if (dolog && logger && lane == 0 && p.logi < c.log_cap) {
+        double* lg = c.prop_log + 4 * p.logi;
+        lg[0] = p_acc;
+        lg[1] = uval;
+        lg[2] = acc;
+        lg[3] = dim / dre;
+      }
+      ++p.logi;
+      if (acc) {
+        ++p.nacc;
+        p.dS_acc += -log(read_lane(exp_dS, 16 * g));
+        const double n0 = read_lane(nw[0], 16 * g), n1 = read_lane(nw[1], 16 * g), n2 = read_lane(nw[2], 16 * g);
+        if (lane == 0) {
+          s.phl[3 * site + 0] = n0;
+          s.phl[3 * site + 1] = n1;
+          s.phl[3 * site + 2] = n2;
+        }
+        // the four neighbours' space sums, recomputed in the original order
+        if (lane < 12) {
+          const int j = s.nbl[(lane / 3) * N + site], a = lane - 3 * (lane / 3);
+          double sn = 0.0;
+          for (int k = 0; k < 4; ++k) sn += s.phl[3 * s.nbl[k * N + j] + a];
+          s.snb[3 * j + a] = sn;
+        }
+        // M^-1 (b, a) = cofactor (a, b) / det M, one entry per lane of row g
+        cplx m3[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) m3[r * 3 + q] = shfl_c(mm, 16 * g + (r + (r >= ea)) * 4 + q + (q >= eb));
+        cplx cof = m3[0] * (m3[4] * m3[8] - m3[5] * m3[7]) - m3[1] * (m3[3] * m3[8] - m3[5] * m3[6]) +
+                   m3[2] * (m3[3] * m3[7] - m3[4] * m3[6]);
+        if ((ea + eb) & 1) cof = -cof;
+        if (row == g) {
+          sh.delta[ea * 4 + ((ea + eb) & 3)] = dm;
+          sh.minv[eb * 4 + ea] = cof * cinv(det);
+        }
+        p.i = site + 1;
+        return site;
+      }
     }
   }
-  p.i = i;
+  p.i = limit;
   return limit;
 }
 
