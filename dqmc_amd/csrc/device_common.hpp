@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace dqmc {
 
 struct cplx {
@@ -86,6 +88,77 @@ __device__ inline cplx wave_sum(cplx v) {
   v.re = wave_sum(v.re);
   v.im = wave_sum(v.im);
   return v;
+}
+
+// static_for<I, N>(f): f(std::integral_constant<int, k>{}) for k = I .. N-1, each k a
+// compile-time constant in f (register arrays indexed by it stay in registers).
+template <int I, int N, class F>
+__device__ inline void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// Sums of many values at once. v_permlane32_swap exchanges the upper half of x with the
+// lower half of y (v_permlane16_swap: the odd rows of x with the even rows of y), so
+// swap_sum<32>(a, b) = a[l] + a[l ^ 32] on lanes 0-31 and b[l] + b[l ^ 32] on lanes 32-63
+// (swap_sum<16>: a on rows 0 and 2, b on rows 1 and 3, each summed with lane l ^ 16).
+template <int S>
+__device__ inline double swap_sum(double a, double b) {
+  const long long ab = __double_as_longlong(a), bb = __double_as_longlong(b);
+  const unsigned alo = (unsigned)(ab & 0xffffffffll), ahi = (unsigned)(ab >> 32);
+  const unsigned blo = (unsigned)(bb & 0xffffffffll), bhi = (unsigned)(bb >> 32);
+  const auto lo = S == 32 ? __builtin_amdgcn_permlane32_swap(alo, blo, false, false)
+                          : __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+  const auto hi = S == 32 ? __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false)
+                          : __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+  const double r0 = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+  const double r1 = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+  return r0 + r1;
+}
+// One butterfly stage inside 16-lane rows: the lane pairs l, l ^ M (CTRL is the DPP
+// control that reads lane l ^ M: row_mirror 15, row_half_mirror 7, quad_perm 3 and 1).
+// With c >= 1 values left per lane, the lane whose bit B is clear keeps the first half
+// and receives its partner's copy of it, the other keeps the second half; with one
+// value left the stage is a plain all-reduce of the pair.
+template <int CTRL, int B, int C>
+__device__ inline void row_stage(double (&v)[C], int lane) {
+  if constexpr (C == 1) {
+    v[0] += dpp_move<CTRL>(v[0]);
+  } else {
+    const bool hi = (lane >> B) & 1;
+#pragma unroll
+    for (int k = 0; k < C / 2; ++k) {
+      const double a = v[k], b = v[k + C / 2];
+      v[k] = (hi ? b : a) + dpp_move<CTRL>(hi ? a : b);
+    }
+  }
+}
+// wave_reduce_scatter: every one of v[0..V-1] (V = 8, 16, 32 or 64) summed over the 64 lanes
+// of the wave by a butterfly reduce-scatter, each stage halving the values a lane carries,
+// so the whole reduction costs about 2 V exchanges instead of 12 V for V separate
+// wave_sums. On return lane l holds the total of value l >> (6 - log2 V) in v[0] (lanes
+// that share it hold the same bits). Needs all 64 lanes active.
+template <int V>
+__device__ inline void wave_reduce_scatter(double (&v)[V]) {
+  static_assert(V == 8 || V == 16 || V == 32 || V == 64, "V");
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < V / 2; ++k) v[k] = swap_sum<32>(v[k], v[k + V / 2]);
+#pragma unroll
+  for (int k = 0; k < V / 4; ++k) v[k] = swap_sum<16>(v[k], v[k + V / 4]);
+  double (&w8)[V / 4] = reinterpret_cast<double (&)[V / 4]>(v);
+  row_stage<0x140, 3>(w8, lane);  // row_mirror: l ^ 15
+  constexpr int C4 = V / 8 > 1 ? V / 8 : 1;
+  double (&w4)[C4] = reinterpret_cast<double (&)[C4]>(v);
+  row_stage<0x141, 2>(w4, lane);  // row_half_mirror: l ^ 7
+  constexpr int C2 = V / 16 > 1 ? V / 16 : 1;
+  double (&w2)[C2] = reinterpret_cast<double (&)[C2]>(v);
+  row_stage<0x1B, 1>(w2, lane);  // quad_perm [3,2,1,0]: l ^ 3
+  constexpr int C1 = V / 32 > 1 ? V / 32 : 1;
+  double (&w1)[C1] = reinterpret_cast<double (&)[C1]>(v);
+  row_stage<0xB1, 0>(w1, lane);  // quad_perm [1,0,3,2]: l ^ 1
 }
 
 // XCD-aware workgroup numbering. The dispatcher hands consecutive workgroups of a launch

@@ -15,9 +15,9 @@
 // qr_Tf for k_qr_solve.
 //
 // Per panel p, with one-panel look-ahead: workgroup 0 applies Q_p^H to panel p+1's columns
-// and factors panel p+1 in LDS (zlarfg per column, applied to the rest of the panel, then
-// zlarft) while the other workgroups apply Q_p^H to the columns beyond (one wave per
-// column); one group barrier per panel.
+// and factors panel p+1 in registers (zlarfg per column, applied to the rest of the panel,
+// with the zlarft inner products formed along the way) while the other workgroups apply
+// Q_p^H to the columns beyond (one wave per column); one group barrier per panel.
 #include "chain.hpp"
 
 namespace dqmc {
@@ -57,32 +57,16 @@ __device__ inline void stage_panel(cplx* P, const cplx* A, const int* perm, int 
   }
 }
 
-template <int NB>
-__device__ inline void unstage_panel(cplx* A, const cplx* P, const int* perm, int k0, int kb, int n, int tid) {
-  cplx* dst[NB];
-#pragma unroll
-  for (int t = 0; t < NB; ++t) dst[t] = A + (size_t)perm[k0 + min(t, kb - 1)] * n;
-  for (int r = k0 + tid; r < n; r += 256) {
-    cplx v[NB];
-#pragma unroll
-    for (int t = 0; t < NB; ++t) v[t] = P[(size_t)min(t, kb - 1) * n + r];
-#pragma unroll
-    for (int t = 0; t < NB; ++t) {  // t >= kb rewrites column kb-1 with its own values
-      dst[t][r].re = v[t].re;  // (a whole-struct copy here is lowered through a stack slot)
-      dst[t][r].im = v[t].im;
-    }
-  }
-}
-
-// Inner loops below are written without data-dependent branches around loads: a guard
-// around each LDS read makes the compiler wait for it before the next one, which
-// serialises the loop on LDS latency. Out-of-range rows and columns are read from
-// clamped in-bounds addresses and their contributions zeroed; masked-off stores go to a
-// per-thread sink slot.
-template <int NB>
+// Workgroup 0's panel work keeps the panel in registers: thread tid owns rows r = tid + 256 q
+// (q < RQ) of the NB panel columns. Every cross-thread sum goes through one butterfly
+// reduce-scatter per wave and one LDS exchange, so a panel column costs one barrier.
+// Selections below are written without data-dependent branches around loads: a guard
+// around each LDS read makes the compiler wait for it before the next one.
+template <int NB, int RQ>
 __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, MatRef Ar, VecRef Dr, int prepivot) {
-  constexpr int KC = (NB + 3) / 4;  // panel columns per wave in the per-column update
-  constexpr int QN = (NB == 16 ? 512 : kQrMaxN) / 64;  // row blocks of a column (NB = 16 only for n <= 512)
+  constexpr int QN = RQ * 4;              // row blocks of 64 in apply_col (n <= 256 RQ)
+  constexpr int V = 2 * NB;               // doubles per butterfly
+  constexpr int LS = NB == 8 ? 2 : 1;     // after it, lane l holds value l >> LS
   const ChainDesc& c = chains[blockIdx.y];
   const int n = c.n;
   const int W = gridDim.x, w = blockIdx.x;
@@ -92,12 +76,14 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   GroupSync g = chain_group(c);
 
   extern __shared__ double smem[];
-  cplx* P = reinterpret_cast<cplx*>(smem);  // [NB][n] panel (rows k0..n-1 used) / staged V
-  cplx* sink = P + (size_t)NB * n;           // [256] one slot per thread
+  cplx* P = reinterpret_cast<cplx*>(smem);  // [NB][n] panel V (rows k0..n-1 used)
   __shared__ int perm[kQrMaxN], lpos[kQrMaxN];
   __shared__ cplx Tl[NB * NB];  // Tl[j * NB + t] = T(j, t)
   __shared__ cplx taul[NB];
   __shared__ double betal[NB];
+  __shared__ double red[2][4][V];     // per-wave sums of a panel step, by step parity
+  __shared__ cplx piv[2][NB];         // the pivot row of a panel step, by step parity
+  __shared__ double redW[4][NB][V];   // per-wave V^H a of the next panel's columns
 
   // ---- column order
   if (prepivot) {
@@ -129,141 +115,151 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   }
   __syncthreads();
 
-  // ---- the panel factorisation of panel p (workgroup 0 alone, in LDS)
-  auto factor_panel = [&](int k0, int p) {
+  cplx x[RQ][NB];  // this thread's rows of the panel being factored
+
+  // rows >= kr of logical columns kc .. kc+kbc-1 into x (everything else zero)
+  auto load_regs = [&](int kc, int kbc, int kr) __attribute__((always_inline)) {
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      const cplx* a = A + (size_t)perm[kc + min(t, kbc - 1)] * n;
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) {
+        const int r = tid + 256 * q;
+        const cplx* src = a + min(r, n - 1);
+        const cplx v = mk(src->re, src->im);
+        x[q][t] = (t < kbc && r >= kr && r < n) ? v : mk(0, 0);
+      }
+    }
+  };
+
+  // ---- factor the panel at logical column k0 (panel index p) held in x; rows >= kw of x are
+  // valid and written back. Per column t (zlarfg, then H^H on the columns after it):
+  //   one butterfly sums |x_t|^2 and x_t^H a_u over the rows below the pivot, plus the zlarft
+  //   products S(j, t-1) = v_j^H v_{t-1}; the pivot row goes through LDS; after one barrier
+  //   every thread forms beta, tau and 1/(alpha - beta) from the same sums and updates its rows:
+  //   v^H a_u = a_u(i) + conj(1/(alpha - beta)) x_t^H a_u.
+  auto factor_regs = [&](int k0, int p, int kw) __attribute__((always_inline)) {
     const int kb = min(NB, n - k0);
     cplx* Tg = c.qr_Tf + (size_t)p * NB * NB;
-    stage_panel<NB>(P, A, perm, k0, kb, n, tid, 0);
-    __syncthreads();
-    // per column: wave 0 forms the reflector (zlarfg) alone, one barrier publishes it, every
-    // wave applies it to its share of the remaining panel columns, one barrier
-    for (int t = 0; t < kb; ++t) {
+    static_for<0, NB>([&](auto tc) __attribute__((always_inline)) {
+      constexpr int t = decltype(tc)::value, par = t & 1;
       const int i = k0 + t;
-      cplx* x = P + (size_t)t * n;
-      if (wave == 0) {
-        double part = 0.0;
-        for (int kq = i + 1 + lane; kq < n; kq += 256) {
-          cplx v[4];
+      double v[V];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) v[q] = x[min(kq + 64 * q, n - 1)];
+      for (int e = 0; e < V; ++e) v[e] = 0.0;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) part += (kq + 64 * q < n) ? abs2(v[q]) : 0.0;
+      for (int q = 0; q < RQ; ++q) {
+        const int r = tid + 256 * q;
+        const cplx xt = (r > i) ? x[q][t] : mk(0, 0);
+        v[0] += abs2(xt);
+#pragma unroll
+        for (int u = t + 1; u < NB; ++u) {
+          const int e = 1 + 2 * (u - t - 1);
+          v[e] = fma(xt.re, x[q][u].re, fma(xt.im, x[q][u].im, v[e]));
+          v[e + 1] = fma(xt.re, x[q][u].im, fma(-xt.im, x[q][u].re, v[e + 1]));
         }
-        const double xnorm = sqrt(wave_sum(part));
-        const cplx alpha = x[i];
-        cplx tau;
-        double beta;
-        // zlarfg: H^H (alpha, x) = (beta, 0), v = (1, x / (alpha - beta))
-        if (xnorm == 0.0 && alpha.im == 0.0) {
-          tau = mk(0, 0);
-          beta = alpha.re;
-        } else {
-          beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm * xnorm), alpha.re);
-          tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
-          const cplx sc = cinv(alpha - mk(beta, 0));
-          for (int kq = i + 1 + lane; kq < n; kq += 256) {
-            cplx v[4];
+        if constexpr (t >= 2) {  // S(j, t-1), j < t-1: rows >= i-1, v_{t-1} has its unit at row i-1
+          const cplx b = (r == i - 1) ? mk(1, 0) : (r > i - 1) ? x[q][t - 1] : mk(0, 0);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = x[min(kq + 64 * q, n - 1)];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int k = kq + 64 * q;
-              *(k < n ? x + k : sink + tid) = v[q] * sc;
-            }
+          for (int j = 0; j < t - 1; ++j) {
+            const int e = 1 + 2 * (NB - 1 - t) + 2 * j;
+            v[e] = fma(x[q][j].re, b.re, fma(x[q][j].im, b.im, v[e]));
+            v[e + 1] = fma(x[q][j].re, b.im, fma(-x[q][j].im, b.re, v[e + 1]));
           }
-        }
-        if (lane == 0) {  // global copies are written after the panel: a store here
-          taul[t] = tau;   // would make every later __syncthreads wait for it
-          betal[t] = beta;
         }
       }
+#pragma unroll
+      for (int q = 0; q < RQ; ++q)
+        if (tid + 256 * q == i) {  // the owner of the pivot row
+#pragma unroll
+          for (int u = t; u < NB; ++u) {
+            piv[par][u].re = x[q][u].re;
+            piv[par][u].im = x[q][u].im;
+          }
+        }
+      wave_reduce_scatter<V>(v);
+      if ((lane & ((1 << LS) - 1)) == 0) red[par][wave][lane >> LS] = v[0];
       __syncthreads();
-      // H^H = I - conj(tau) v v^H on the remaining panel columns: wave w takes columns
-      // c0 = t+1+w, c0+4, ... (nu of them, wave-uniform); their loads and sums overlap
-      const int c0 = t + 1 + wave;
-      const int nu = c0 < kb ? (kb - c0 + 3) >> 2 : 0;
-      if (nu > 0) {
-        const cplx ctau = conj(taul[t]);
-        cplx* col[KC];
+      const double xsq = (red[par][0][0] + red[par][1][0]) + (red[par][2][0] + red[par][3][0]);
+      const cplx alpha = mk(piv[par][t].re, piv[par][t].im);
+      cplx tau, sc;
+      double beta;
+      // zlarfg: H^H (alpha, x) = (beta, 0), v = (1, x / (alpha - beta))
+      if (xsq == 0.0 && alpha.im == 0.0) {
+        tau = mk(0, 0);
+        beta = alpha.re;
+        sc = mk(1, 0);
+      } else {
+        beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xsq), alpha.re);
+        tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+        sc = cinv(alpha - mk(beta, 0));
+      }
+      // lane k forms f for column t+1+k (its sums read by lane), then every lane takes them
+      // from scalar registers
+      cplx fl;
+      {
+        const int k = min(lane, NB - 1 - t > 0 ? NB - 2 - t : 0), e = 1 + 2 * k;
+        const cplx raw = mk((red[par][0][e] + red[par][1][e]) + (red[par][2][e] + red[par][3][e]),
+                            (red[par][0][e + 1] + red[par][1][e + 1]) + (red[par][2][e + 1] + red[par][3][e + 1]));
+        const int u = min(t + 1 + k, NB - 1);
+        cplx dot = mk(piv[par][u].re, piv[par][u].im);
+        cfma_cj(dot, sc, raw);
+        fl = conj(tau) * dot;
+      }
+      cplx f[NB];
 #pragma unroll
-        for (int u = 0; u < KC; ++u) col[u] = (u < nu) ? P + (size_t)(c0 + 4 * u) * n : x;
-        cplx acc[KC];
+      for (int u = t + 1; u < NB; ++u) f[u] = read_lane_c(fl, u - t - 1);
 #pragma unroll
-        for (int u = 0; u < KC; ++u) acc[u] = mk(0, 0);
-        for (int kq = i + lane; kq < n; kq += 128) {
-          cplx vk[2], pv[2][KC];
+      for (int q = 0; q < RQ; ++q) {
+        const int r = tid + 256 * q;
+        const cplx xt = x[q][t];
+        const cplx vt = (r > i) ? sc * xt : (r == i) ? mk(1, 0) : mk(0, 0);
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const int kk = min(kq + 64 * q, n - 1);
-            vk[q] = x[kk];
+        for (int u = t + 1; u < NB; ++u) x[q][u] = x[q][u] - f[u] * vt;
+        x[q][t] = (r > i) ? vt : (r == i) ? mk(beta, 0) : xt;
+      }
+      if (t >= 2 && wave == 0 && lane < t - 1) {  // lane j: S(j, t-1)
+        const int e = 1 + 2 * (NB - 1 - t) + 2 * lane;
+        Tl[lane * NB + t - 1] = mk((red[par][0][e] + red[par][1][e]) + (red[par][2][e] + red[par][3][e]),
+                                   (red[par][0][e + 1] + red[par][1][e + 1]) + (red[par][2][e + 1] + red[par][3][e + 1]));
+      }
+      if (tid == 0) {
+        taul[t] = tau;
+        betal[t] = beta;
+      }
+    });
+    {  // S(j, NB-1), j < NB-1
+      constexpr int par = NB & 1;
+      double v[V];
 #pragma unroll
-            for (int u = 0; u < KC; ++u) pv[q][u] = col[u][kk];
-          }
+      for (int e = 0; e < V; ++e) v[e] = 0.0;
+      const int rp = k0 + NB - 1;
 #pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const int k = kq + 64 * q;
-            const cplx v = (k == i) ? mk(1, 0) : (k < n) ? vk[q] : mk(0, 0);
+      for (int q = 0; q < RQ; ++q) {
+        const int r = tid + 256 * q;
+        const cplx b = (r == rp) ? mk(1, 0) : (r > rp) ? x[q][NB - 1] : mk(0, 0);
 #pragma unroll
-            for (int u = 0; u < KC; ++u) cfma_cj(acc[u], v, pv[q][u]);
-          }
+        for (int j = 0; j < NB - 1; ++j) {
+          v[2 * j] = fma(x[q][j].re, b.re, fma(x[q][j].im, b.im, v[2 * j]));
+          v[2 * j + 1] = fma(x[q][j].re, b.im, fma(-x[q][j].im, b.re, v[2 * j + 1]));
         }
-        cplx f[KC];
-#pragma unroll
-        for (int u = 0; u < KC; ++u) f[u] = (u < nu) ? ctau * wsum(acc[u]) : mk(0, 0);
-        for (int kq = i + lane; kq < n; kq += 128) {
-          cplx vk[2], pv[2][KC];
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const int kk = min(kq + 64 * q, n - 1);
-            vk[q] = x[kk];
-#pragma unroll
-            for (int u = 0; u < KC; ++u) pv[q][u] = col[u][kk];
-          }
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const int k = kq + 64 * q;
-            const cplx v = (k == i) ? mk(1, 0) : vk[q];
-#pragma unroll
-            for (int u = 0; u < KC; ++u) *((u < nu && k < n) ? col[u] + k : sink + tid) = pv[q][u] - f[u] * v;
-          }
-        }
+      }
+      wave_reduce_scatter<V>(v);
+      if ((lane & ((1 << LS) - 1)) == 0) red[par][wave][lane >> LS] = v[0];
+      __syncthreads();
+      if (wave == 0 && lane < NB - 1) {
+        const int e = 2 * lane;
+        Tl[lane * NB + NB - 1] = mk((red[par][0][e] + red[par][1][e]) + (red[par][2][e] + red[par][3][e]),
+                                    (red[par][0][e + 1] + red[par][1][e + 1]) + (red[par][2][e + 1] + red[par][3][e + 1]));
       }
       __syncthreads();
     }
-    // R's diagonal goes where the reflectors' implicit ones were (never read again)
     if (tid < kb) {
-      P[(size_t)tid * n + k0 + tid] = mk(betal[tid], 0);
       D[k0 + tid] = fabs(betal[tid]);
       c.qr_tau[k0 + tid] = taul[tid];
     }
-    // ---- zlarft (forward, columnwise): T(t,t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) V(:, 0:t)^H v_t
-    // first the inner products S(j, t) = V(:, j)^H v_t, j < t, into Tl(j, t); wave w takes
-    // t = 1+w, 5+w, ...; the products of one t share v_t's loads. Columns j >= t are read
-    // too (in bounds) and their sums dropped.
-    for (int t = 1 + wave; t < kb; t += nwaves) {
-      const cplx* vt = P + (size_t)t * n;
-      const int it = k0 + t;
-      cplx acc[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) acc[j] = mk(0, 0);
-      for (int k = it + lane; k < n; k += 64) {
-        const cplx b = (k == it) ? mk(1, 0) : vt[k];
-        cplx pj[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + k];  // rows >= it: below v_j's diagonal
-#pragma unroll
-        for (int j = 0; j < NB; ++j) cfma_cj(acc[j], pj[j], b);
-      }
-#pragma unroll
-      for (int j = 0; j < NB; ++j)
-        if (j < t) {
-          const cplx sj = wsum(acc[j]);
-          if (lane == 0) Tl[j * NB + t] = sj;
-        }
-    }
-    __syncthreads();
-    // the triangle, by wave 0: lane j keeps row j of T in registers; column t needs
-    // sum_{l<t} T(j, l) S(l, t), where T(j, l) = 0 for j > l
+    // ---- zlarft (forward, columnwise) from S: T(t,t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) S(0:t, t);
+    // wave 0, lane j keeps row j of T in registers (T(j, l) = 0 for j > l)
     if (wave == 0) {
       cplx trow[NB];
 #pragma unroll
@@ -286,9 +282,93 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
         }
       }
     }
+    // x back to A (rows >= kw: R above and on the diagonal, reflectors below) and, as the
+    // unit lower trapezoid V, to P for the next panel's update
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      cplx* a = A + (size_t)perm[k0 + min(t, kb - 1)] * n;
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) {
+        const int r = tid + 256 * q;
+        const cplx xv = x[q][t];
+        if (r >= k0 && r < n) {
+          const cplx pv = (t >= kb || r < k0 + t) ? mk(0, 0) : (r == k0 + t) ? mk(1, 0) : xv;
+          P[(size_t)t * n + r].re = pv.re;
+          P[(size_t)t * n + r].im = pv.im;
+        }
+        if (t < kb && r >= kw && r < n) {
+          a[r].re = xv.re;
+          a[r].im = xv.im;
+        }
+      }
+    }
     __syncthreads();
-    unstage_panel<NB>(A, P, perm, k0, kb, n, tid);
   };
+
+  // ---- x <- (I - V T^H V^H) x for the next panel (logical columns k1 .. k1+kb1-1, rows >= k0),
+  // with V (rows >= k0) in P and T in Tl: W = V^H x by one butterfly per column, then every
+  // wave forms Z = T^H W itself (entry u NB + j on lane (u NB + j) mod 64) and the row
+  // updates take Z from scalar registers
+  auto apply_next_regs = [&](int k0, int k1, int kb1) __attribute__((always_inline)) {
+    constexpr int ZE = NB * NB / 64;  // Z entries per lane
+    load_regs(k1, kb1, k0);
+    cplx pj[RQ][NB];
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      const int r = tid + 256 * q;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const cplx* src = P + (size_t)j * n + min(r, n - 1);
+        const cplx pv = mk(src->re, src->im);
+        pj[q][j] = (r >= k0 && r < n) ? pv : mk(0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      double v[V];
+#pragma unroll
+      for (int e = 0; e < V; ++e) v[e] = 0.0;
+#pragma unroll
+      for (int q = 0; q < RQ; ++q)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          v[2 * j] = fma(pj[q][j].re, x[q][u].re, fma(pj[q][j].im, x[q][u].im, v[2 * j]));
+          v[2 * j + 1] = fma(pj[q][j].re, x[q][u].im, fma(-pj[q][j].im, x[q][u].re, v[2 * j + 1]));
+        }
+      wave_reduce_scatter<V>(v);
+      if ((lane & ((1 << LS) - 1)) == 0) redW[wave][u][lane >> LS] = v[0];
+    }
+    __syncthreads();
+    cplx zr[ZE];
+#pragma unroll
+    for (int m = 0; m < ZE; ++m) {
+      const int idx = lane + 64 * m, u = idx / NB, j = idx % NB;
+      cplx z = mk(0, 0);
+#pragma unroll
+      for (int l = 0; l < NB; ++l) {
+        const cplx wl = mk((redW[0][u][2 * l] + redW[1][u][2 * l]) + (redW[2][u][2 * l] + redW[3][u][2 * l]),
+                           (redW[0][u][2 * l + 1] + redW[1][u][2 * l + 1]) +
+                               (redW[2][u][2 * l + 1] + redW[3][u][2 * l + 1]));
+        cfma_cj(z, Tl[l * NB + j], wl);
+      }
+      zr[m] = z;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int idx = u * NB + j;
+        const cplx z = read_lane_c(zr[idx / 64], idx % 64);
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+          cplx a = x[q][u];
+          a.re = fma(-pj[q][j].re, z.re, fma(pj[q][j].im, z.im, a.re));
+          a.im = fma(-pj[q][j].re, z.im, fma(-pj[q][j].im, z.re, a.im));
+          x[q][u] = a;
+        }
+      }
+  };
+
   // ---- a <- (I - V T^H V^H) a for one column, with panel V (from row k0) staged in P and T in Tl
   auto apply_col = [&](int col, int k0) {
     cplx* a = A + (size_t)col * n;
@@ -343,24 +423,29 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   // One-panel look-ahead: while every other workgroup applies panel p to the columns beyond
   // panel p+1, workgroup 0 applies it to panel p+1's columns and factors panel p+1, so each
   // panel costs one group barrier and its factorisation overlaps the trailing update.
-  if (w == 0) factor_panel(0, 0);
+  // Workgroup 0 keeps V and T of the panel it factored in P and Tl; the others stage them.
+  if (w == 0) {
+    load_regs(0, min(NB, n), 0);
+    factor_regs(0, 0, 0);
+  }
   group_sync(g);
   if (group_failed(g)) return;
   for (int k0 = 0, p = 0; k0 + NB < n; k0 += NB, ++p) {  // panel p (full width) is factored
     const int k1 = k0 + NB, kb1 = min(NB, n - k1);
-    stage_panel<NB>(P, A, perm, k0, NB, n, tid, 1);
-    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = c.qr_Tf[(size_t)p * NB * NB + e];
-    __syncthreads();
-    if (w == 0) {
-      for (int u = wave; u < kb1; u += nwaves) apply_col(perm[k1 + u], k0);
-      if (W == 1)  // nobody else to take the trailing columns
-        for (int col = wave; col < n; col += nwaves)
-          if (lpos[col] >= k1 + NB) apply_col(col, k0);
+    if (w != 0) {
+      stage_panel<NB>(P, A, perm, k0, NB, n, tid, 1);
+      for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = c.qr_Tf[(size_t)p * NB * NB + e];
       __syncthreads();
-      factor_panel(k1, p + 1);
-    } else {
       for (int col = (w - 1) + (W - 1) * wave; col < n; col += (W - 1) * nwaves)
         if (lpos[col] >= k1 + NB) apply_col(col, k0);
+    } else {
+      apply_next_regs(k0, k1, kb1);
+      if (W == 1) {  // nobody else to take the trailing columns
+        for (int col = wave; col < n; col += nwaves)
+          if (lpos[col] >= k1 + NB) apply_col(col, k0);
+        __syncthreads();
+      }
+      factor_regs(k1, p + 1, k0);
     }
     group_sync(g);
     if (group_failed(g)) return;
@@ -624,19 +709,20 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
 
 int qr_panel_width(int nmax) { return nmax <= 512 ? 16 : 8; }
 
-template <int NB>
+template <int NB, int RQ>
 hipError_t launch_qr_blocked_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
                                 int prepivot, int wg) {
-  const size_t lds = ((size_t)NB * nmax + 256) * sizeof(cplx);
+  const size_t lds = (size_t)NB * nmax * sizeof(cplx);
   static size_t attr = 0;
   if (lds > 65536 && lds > attr) {
     hipError_t e =
-        hipFuncSetAttribute((const void*)k_qr_blocked<NB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipFuncSetAttribute((const void*)k_qr_blocked<NB, RQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     attr = lds;
   }
   void* args[] = {(void*)&chains, (void*)&A, (void*)&D, (void*)&prepivot};
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_qr_blocked<NB>, dim3(wg, nchains), dim3(256), args, lds, s);
+  hipError_t e =
+      hipLaunchCooperativeKernel((const void*)k_qr_blocked<NB, RQ>, dim3(wg, nchains), dim3(256), args, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
   return e;
 }
@@ -645,8 +731,9 @@ hipError_t launch_qr_blocked(hipStream_t s, const ChainDesc* chains, int nchains
                              int prepivot, int wg, int nb) {
   if (nmax > kQrMaxN) return hipErrorInvalidValue;
   if (nb == 0) nb = qr_panel_width(nmax);
-  if (nb == 16 && nmax <= 512) return launch_qr_blocked_nb<16>(s, chains, nchains, nmax, A, D, prepivot, wg);
-  if (nb == 8) return launch_qr_blocked_nb<8>(s, chains, nchains, nmax, A, D, prepivot, wg);
+  if (nb == 16 && nmax <= 512) return launch_qr_blocked_nb<16, 2>(s, chains, nchains, nmax, A, D, prepivot, wg);
+  if (nb == 8 && nmax <= 512) return launch_qr_blocked_nb<8, 2>(s, chains, nchains, nmax, A, D, prepivot, wg);
+  if (nb == 8) return launch_qr_blocked_nb<8, 4>(s, chains, nchains, nmax, A, D, prepivot, wg);
   return hipErrorInvalidValue;
 }
 
