@@ -57,6 +57,26 @@ __host__ __device__ inline cplx cinv(cplx a) {
   return cplx{r / d, -1.0 / d};
 }
 
+// Global-memory views. Buffer pointers read from a descriptor are generic, so accesses
+// through them compile to FLAT instructions, which count in LGKM_CNT as well as VM_CNT:
+// every wait for an LDS result, and every barrier, then also waits for the outstanding
+// global stores. Through these address_space(1) views the same accesses compile to GLOBAL
+// instructions, which only count in VM_CNT.
+typedef __attribute__((address_space(1))) cplx gcplx;
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) int gint;
+__device__ inline gcplx* gp(cplx* p) { return (gcplx*)p; }
+__device__ inline const gcplx* gp(const cplx* p) { return (const gcplx*)p; }
+__device__ inline gdouble* gp(double* p) { return (gdouble*)p; }
+__device__ inline const gdouble* gp(const double* p) { return (const gdouble*)p; }
+__device__ inline gint* gp(int* p) { return (gint*)p; }
+__device__ inline const gint* gp(const int* p) { return (const gint*)p; }
+__device__ inline cplx ld(const gcplx* p) { return mk(p->re, p->im); }
+__device__ inline void st(gcplx* p, cplx v) {
+  p->re = v.re;
+  p->im = v.im;
+}
+
 // ---------------------------------------------------------------------------
 // Reductions (wave64)
 //

@@ -36,9 +36,9 @@ __device__ inline cplx read_lane_c(cplx v, int lane) { return mk(read_lane(v.re,
 // mode 0: P <- A as is; mode 1: P <- unit lower trapezoid (V with its implicit ones and
 // zeros above the diagonal), columns t >= kb zero-filled.
 template <int NB>
-__device__ inline void stage_panel(cplx* P, const cplx* A, const int* perm, int k0, int kb, int n, int tid,
+__device__ inline void stage_panel(cplx* P, const gcplx* A, const int* perm, int k0, int kb, int n, int tid,
                                    int mode) {
-  const cplx* src[NB];
+  const gcplx* src[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
     const int k = k0 + min(t, kb - 1);
@@ -47,7 +47,7 @@ __device__ inline void stage_panel(cplx* P, const cplx* A, const int* perm, int 
   for (int r = k0 + tid; r < n; r += 256) {
     cplx v[NB];
 #pragma unroll
-    for (int t = 0; t < NB; ++t) v[t] = src[t][r];
+    for (int t = 0; t < NB; ++t) v[t] = ld(src[t] + r);
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
       cplx val = v[t];
@@ -64,15 +64,14 @@ __device__ inline void stage_panel(cplx* P, const cplx* A, const int* perm, int 
 // around each LDS read makes the compiler wait for it before the next one.
 template <int NB, int RQ>
 __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, MatRef Ar, VecRef Dr, int prepivot) {
-  constexpr int QN = RQ * 4;              // row blocks of 64 in apply_col (n <= 256 RQ)
   constexpr int V = 2 * NB;               // doubles per butterfly
   constexpr int LS = NB == 8 ? 2 : 1;     // after it, lane l holds value l >> LS
   const ChainDesc& c = chains[blockIdx.y];
   const int n = c.n;
   const int W = gridDim.x, w = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-  cplx* A = matptr(c, Ar);
-  double* D = vecptr(c, Dr);
+  gcplx* A = gp(matptr(c, Ar));
+  gdouble* D = gp(vecptr(c, Dr));
   GroupSync g = chain_group(c);
 
   extern __shared__ double smem[];
@@ -83,21 +82,21 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   __shared__ double betal[NB];
   __shared__ double red[2][4][V];     // per-wave sums of a panel step, by step parity
   __shared__ cplx piv[2][NB];         // the pivot row of a panel step, by step parity
-  __shared__ double redW[4][NB][V];   // per-wave V^H a of the next panel's columns
+  __shared__ double redW[4][NB][V + 1];  // per-wave V^H a of NB columns (padded: no bank conflicts)
 
   // ---- column order
   if (prepivot) {
     for (int col = w + W * wave; col < n; col += W * nwaves) {
-      const cplx* a = A + (size_t)col * n;
+      const gcplx* a = A + (size_t)col * n;
       double acc = 0.0;
-      for (int k = lane; k < n; k += 64) acc += abs2(a[k]);
+      for (int k = lane; k < n; k += 64) acc += abs2(ld(a + k));
       acc = wave_sum(acc);
-      if (lane == 0) c.qr_vn[col] = sqrt(acc);
+      if (lane == 0) gp(c.qr_vn)[col] = sqrt(acc);
     }
     group_sync(g);
     if (group_failed(g)) return;
     double* vn = reinterpret_cast<double*>(P);  // n doubles, before the panel buffer is used
-    for (int k = tid; k < n; k += blockDim.x) vn[k] = c.qr_vn[k];
+    for (int k = tid; k < n; k += blockDim.x) vn[k] = gp(c.qr_vn)[k];
     __syncthreads();
     for (int col = tid; col < n; col += blockDim.x) {
       const double v = vn[col];
@@ -121,11 +120,11 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   auto load_regs = [&](int kc, int kbc, int kr) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      const cplx* a = A + (size_t)perm[kc + min(t, kbc - 1)] * n;
+      const gcplx* a = A + (size_t)perm[kc + min(t, kbc - 1)] * n;
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
         const int r = tid + 256 * q;
-        const cplx* src = a + min(r, n - 1);
+        const gcplx* src = a + min(r, n - 1);
         const cplx v = mk(src->re, src->im);
         x[q][t] = (t < kbc && r >= kr && r < n) ? v : mk(0, 0);
       }
@@ -138,9 +137,11 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   //   products S(j, t-1) = v_j^H v_{t-1}; the pivot row goes through LDS; after one barrier
   //   every thread forms beta, tau and 1/(alpha - beta) from the same sums and updates its rows:
   //   v^H a_u = a_u(i) + conj(1/(alpha - beta)) x_t^H a_u.
+  // Every call site of this and of the block update below is single, and the kernel's code
+  // fits the instruction cache: the panel loop runs without instruction fetch stalls.
   auto factor_regs = [&](int k0, int p, int kw) __attribute__((always_inline)) {
     const int kb = min(NB, n - k0);
-    cplx* Tg = c.qr_Tf + (size_t)p * NB * NB;
+    gcplx* Tg = gp(c.qr_Tf) + (size_t)p * NB * NB;
     static_for<0, NB>([&](auto tc) __attribute__((always_inline)) {
       constexpr int t = decltype(tc)::value, par = t & 1;
       const int i = k0 + t;
@@ -191,8 +192,11 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
         sc = mk(1, 0);
       } else {
         beta = -copysign(sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xsq), alpha.re);
-        tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
-        sc = cinv(alpha - mk(beta, 0));
+        const double ib = 1.0 / beta;  // two divisions on the per-column critical path, not five
+        tau = mk((beta - alpha.re) * ib, -alpha.im * ib);
+        const cplx d = alpha - mk(beta, 0);  // |d|^2 <= 4 (|alpha|^2 + xsq): no new overflow
+        const double id = 1.0 / (d.re * d.re + d.im * d.im);
+        sc = mk(d.re * id, -d.im * id);
       }
       // lane k forms f for column t+1+k (its sums read by lane), then every lane takes them
       // from scalar registers
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
     }
     if (tid < kb) {
       D[k0 + tid] = fabs(betal[tid]);
-      c.qr_tau[k0 + tid] = taul[tid];
+      st(gp(c.qr_tau) + k0 + tid, taul[tid]);
     }
     // ---- zlarft (forward, columnwise) from S: T(t,t) = tau_t, T(0:t, t) = -tau_t T(0:t, 0:t) S(0:t, t);
     // wave 0, lane j keeps row j of T in registers (T(j, l) = 0 for j > l)
@@ -278,7 +282,7 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
 #pragma unroll
         for (int t = 0; t < NB; ++t) {
           Tl[lane * NB + t] = trow[t];
-          Tg[lane * NB + t] = trow[t];
+          st(Tg + lane * NB + t, trow[t]);
         }
       }
     }
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
     // unit lower trapezoid V, to P for the next panel's update
 #pragma unroll
     for (int t = 0; t < NB; ++t) {
-      cplx* a = A + (size_t)perm[k0 + min(t, kb - 1)] * n;
+      gcplx* a = A + (size_t)perm[k0 + min(t, kb - 1)] * n;
 #pragma unroll
       for (int q = 0; q < RQ; ++q) {
         const int r = tid + 256 * q;
@@ -301,18 +305,15 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
           a[r].im = xv.im;
         }
       }
-    }
-    __syncthreads();
+    }  // the group barrier that follows every call publishes P, Tl and A
   };
 
-  // ---- x <- (I - V T^H V^H) x for the next panel (logical columns k1 .. k1+kb1-1, rows >= k0),
-  // with V (rows >= k0) in P and T in Tl: W = V^H x by one butterfly per column, then every
-  // wave forms Z = T^H W itself (entry u NB + j on lane (u NB + j) mod 64) and the row
-  // updates take Z from scalar registers
-  auto apply_next_regs = [&](int k0, int k1, int kb1) __attribute__((always_inline)) {
-    constexpr int ZE = NB * NB / 64;  // Z entries per lane
-    load_regs(k1, kb1, k0);
-    cplx pj[RQ][NB];
+  // ---- x <- (I - V T^H V^H) x for NB columns held in x (rows >= k0), with V (rows >= k0) in P,
+  // this thread's rows of it in pj, and T in Tl: W = V^H x by one butterfly per column, then
+  // every wave forms Z = T^H W itself (entry u NB + j on lane (u NB + j) mod 64) and the row
+  // updates take Z from scalar registers. V is read from LDS once per panel, not per column.
+  cplx pj[RQ][NB];
+  auto load_pj = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int q = 0; q < RQ; ++q) {
       const int r = tid + 256 * q;
@@ -323,6 +324,9 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
         pj[q][j] = (r >= k0 && r < n) ? pv : mk(0, 0);
       }
     }
+  };
+  auto apply_regs = [&]() __attribute__((always_inline)) {
+    constexpr int ZE = NB * NB / 64;  // Z entries per lane
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
       double v[V];
@@ -368,92 +372,62 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
         }
       }
   };
-
-  // ---- a <- (I - V T^H V^H) a for one column, with panel V (from row k0) staged in P and T in Tl
-  auto apply_col = [&](int col, int k0) {
-    cplx* a = A + (size_t)col * n;
-    // the column's rows k0.. in registers (all loads in flight together)
-    cplx av[QN];
+  // rows >= kr of x back to logical columns kc .. kc+kbc-1
+  auto store_regs = [&](int kc, int kbc, int kr) __attribute__((always_inline)) {
 #pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      const int k = k0 + lane + 64 * q;
-      av[q] = k < n ? a[k] : mk(0, 0);
-    }
-    cplx wv[NB];
+    for (int t = 0; t < NB; ++t) {
+      gcplx* a = A + (size_t)perm[kc + min(t, kbc - 1)] * n;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) wv[j] = mk(0, 0);
-#pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      const int k = k0 + lane + 64 * q;
-      if (k0 + 64 * q < n) {  // uniform
-        const int kk = min(k, n - 1);
-        cplx pj[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + kk];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], av[q]);  // av = 0 past row n-1
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < NB; ++j) wv[j] = wsum(wv[j]);
-    // z = T^H w: lane j < NB forms z_j, then every lane reads all NB of them
-    const int jl = lane & (NB - 1);
-    cplx zl = mk(0, 0);
-#pragma unroll
-    for (int l = 0; l < NB; ++l) cfma_cj(zl, Tl[l * NB + jl], wv[l]);
-    cplx z[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) z[j] = read_lane_c(zl, j);
-#pragma unroll
-    for (int q = 0; q < QN; ++q) {
-      const int k = k0 + lane + 64 * q;
-      if (k0 + 64 * q < n) {
-        const int kk = min(k, n - 1);
-        cplx pj[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) pj[j] = P[(size_t)j * n + kk];
-        cplx ak = av[q];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) ak = ak - pj[j] * z[j];
-        if (k < n) a[k] = ak;
+      for (int q = 0; q < RQ; ++q) {
+        const int r = tid + 256 * q;
+        if (t < kbc && r >= kr && r < n) {
+          a[r].re = x[q][t].re;
+          a[r].im = x[q][t].im;
+        }
       }
     }
   };
-
   // One-panel look-ahead: while every other workgroup applies panel p to the columns beyond
-  // panel p+1, workgroup 0 applies it to panel p+1's columns and factors panel p+1, so each
-  // panel costs one group barrier and its factorisation overlaps the trailing update.
-  // Workgroup 0 keeps V and T of the panel it factored in P and Tl; the others stage them.
-  if (w == 0) {
-    load_regs(0, min(NB, n), 0);
-    factor_regs(0, 0, 0);
-  }
-  group_sync(g);
-  if (group_failed(g)) return;
-  for (int k0 = 0, p = 0; k0 + NB < n; k0 += NB, ++p) {  // panel p (full width) is factored
-    const int k1 = k0 + NB, kb1 = min(NB, n - k1);
-    if (w != 0) {
-      stage_panel<NB>(P, A, perm, k0, NB, n, tid, 1);
-      for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = c.qr_Tf[(size_t)p * NB * NB + e];
-      __syncthreads();
-      for (int col = (w - 1) + (W - 1) * wave; col < n; col += (W - 1) * nwaves)
-        if (lpos[col] >= k1 + NB) apply_col(col, k0);
-    } else {
-      apply_next_regs(k0, k1, kb1);
-      if (W == 1) {  // nobody else to take the trailing columns
-        for (int col = wave; col < n; col += nwaves)
-          if (lpos[col] >= k1 + NB) apply_col(col, k0);
+  // panel p+1 (in blocks of NB logical columns, block b to workgroup 1 + b mod (W-1)),
+  // workgroup 0 applies it to panel p+1's columns and factors panel p+1, so each panel costs
+  // one group barrier and its factorisation overlaps the trailing update. Workgroup 0 keeps
+  // V and T of the panel it factored in P and Tl; the others stage them. Iteration p = -1
+  // factors panel 0 alone. Each code path has one call site, for the instruction cache.
+  const int ws = W == 1 ? 1 : W - 1, w0 = W == 1 ? 0 : 1;
+  for (int p = -1, k1 = 0; k1 < n; ++p, k1 += NB) {  // apply panel p, factor panel p+1 at k1
+    const int k0 = k1 - NB;
+    if (p >= 0) {
+      if (w != 0) {
+        stage_panel<NB>(P, A, perm, k0, NB, n, tid, 1);
+        for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = ld(gp(c.qr_Tf) + (size_t)p * NB * NB + e);
         __syncthreads();
       }
-      factor_regs(k1, p + 1, k0);
+      load_pj(k0);
     }
+    // this workgroup's blocks: trailing ones first (kc >= k1 + NB), then, on workgroup 0,
+    // panel p+1 itself, which stays in x for the factorisation
+    if (p < 0) {
+      if (w == 0) load_regs(0, min(NB, n), 0);
+    } else {
+      for (int kc = (w >= w0) ? k1 + NB + (w - w0) * NB : n;; kc += ws * NB) {
+        const bool next = kc >= n;
+        if (next && w != 0) break;
+        const int kcc = next ? k1 : kc, kbc = min(NB, n - kcc);
+        load_regs(kcc, kbc, k0);
+        apply_regs();
+        if (next) break;
+        store_regs(kcc, kbc, k0);
+        __syncthreads();  // redW is reused by the next block
+      }
+    }
+    if (w == 0) factor_regs(k1, p + 1, p >= 0 ? k0 : 0);
     group_sync(g);
     if (group_failed(g)) return;
   }
   if (w == 0) {
     for (int k = tid; k < n; k += blockDim.x) {
-      c.qr_perm[k] = perm[k];
-      c.qr_lpos[k] = lpos[k];
+      gp(c.qr_perm)[k] = perm[k];
+      gp(c.qr_lpos)[k] = lpos[k];
     }
   }
 }
@@ -476,8 +450,8 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
   const int ncol = min(CB, n - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int jl = wave % CB, rs = wave / CB;
-  const cplx* A = matptr(c, Ar);
-  cplx* B = matptr(c, Br);
+  const gcplx* A = gp(matptr(c, Ar));
+  gcplx* B = gp(matptr(c, Br));
   extern __shared__ double smem[];
   cplx* bc = reinterpret_cast<cplx*>(smem);  // [CB][n]
   cplx* V = bc + (size_t)CB * n;              // [NB][n]
@@ -488,7 +462,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
     cplx v[CB];
 #pragma unroll
     for (int q = 0; q < CB; ++q) {
-      const cplx* src = B + (size_t)(j0 + min(q, ncol - 1)) * n + r;
+      const gcplx* src = B + (size_t)(j0 + min(q, ncol - 1)) * n + r;
       v[q] = mk(src->re, src->im);  // (a whole-struct copy here is lowered through a stack slot)
     }
 #pragma unroll
@@ -500,7 +474,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
     const int kb = min(NB, n - k0);
     __syncthreads();
     stage_panel<NB>(V, A, nullptr, k0, kb, n, tid, 1);  // columns >= kb zero-filled
-    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = c.qr_Tf[(size_t)p * NB * NB + e];
+    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = ld(gp(c.qr_Tf) + (size_t)p * NB * NB + e);
     __syncthreads();
     // w = V^H a over this wave's row slice (rows k0 + lane + 64 (rs + RS m))
     cplx wv[NB];
@@ -559,7 +533,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
     __syncthreads();
     {
       const int kk = tid >> 4, rr = tid & 15;
-      Rd[tid] = (kk < bs && rr < bs) ? A[(size_t)(lo + kk) * n + lo + rr] : mk(0, 0);  // Rd[kk*16+rr] = R(lo+rr, lo+kk)
+      Rd[tid] = (kk < bs && rr < bs) ? ld(A + (size_t)(lo + kk) * n + lo + rr) : mk(0, 0);  // Rd[kk*16+rr] = R(lo+rr, lo+kk)
     }
     __syncthreads();
     if (wave < ncol) {
@@ -577,7 +551,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
     for (int r = tid; r < lo; r += 256) {
       cplx rv[16];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) rv[k] = A[(size_t)(lo + min(k, bs - 1)) * n + r];
+      for (int k = 0; k < 16; ++k) rv[k] = ld(A + (size_t)(lo + min(k, bs - 1)) * n + r);
 #pragma unroll
       for (int q = 0; q < CB; ++q) {
         cplx* y = bc + (size_t)q * n;
@@ -599,7 +573,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
 #pragma unroll
     for (int q = 0; q < CB; ++q)
       if (q < ncol) {
-        cplx* d = B + (size_t)(j0 + q) * n + r;
+        gcplx* d = B + (size_t)(j0 + q) * n + r;
         d->re = v[q].re;
         d->im = v[q].im;
       }
@@ -623,8 +597,8 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
   const int ncol = min(CB, n - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int jl = wave % CB, rs = wave / CB;
-  const cplx* A = matptr(c, Ar);
-  cplx* Q = matptr(c, Qr);
+  const gcplx* A = gp(matptr(c, Ar));
+  gcplx* Q = gp(matptr(c, Qr));
   extern __shared__ double smem[];
   cplx* xc = reinterpret_cast<cplx*>(smem);  // [CB][n]
   cplx* V = xc + (size_t)CB * n;              // [NB][n]
@@ -643,7 +617,7 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
     const int k0 = p * NB, kb = min(NB, n - k0);
     __syncthreads();
     stage_panel<NB>(V, A, c.qr_perm, k0, kb, n, tid, 1);  // columns >= kb zero-filled
-    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = c.qr_Tf[(size_t)p * NB * NB + e];
+    for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = ld(gp(c.qr_Tf) + (size_t)p * NB * NB + e);
     __syncthreads();
     cplx wv[NB];
 #pragma unroll
@@ -700,7 +674,7 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
 #pragma unroll
     for (int q = 0; q < CB; ++q)
       if (q < ncol) {
-        cplx* d = Q + (size_t)(j0 + q) * n + r;
+        gcplx* d = Q + (size_t)(j0 + q) * n + r;
         d->re = v[q].re;
         d->im = v[q].im;
       }
