@@ -19,12 +19,17 @@
 
 namespace dqmc {
 
-// coefficients of interaction_matrix_exp_op (interactions.jl:102-140)
+// coefficients of interaction_matrix_exp_op (interactions.jl:102-140): C = cosh(x),
+// sh = power sinh(x) / |o|, x = lam dtau |o| >= 0. Both from one expm1 and one reciprocal
+// (em = e^x - 1: cosh x = 1 + em^2 / (2 e^x), sinh x = em (1 + e^-x) / 2), accurate for
+// small x too; this sits on the critical path of every proposal.
 __device__ inline void ev_coeffs_op(double lam_dtau, double o0, double o1, double o2, double power, double& C,
                                     cplx& S, double& R) {
   const double nrm = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
-  const double sh = power * sinh(lam_dtau * nrm) / nrm;
-  C = cosh(lam_dtau * nrm);
+  const double em = expm1(lam_dtau * nrm);
+  const double ie = 1.0 / (1.0 + em);
+  C = fma(0.5 * em, em * ie, 1.0);
+  const double sh = power * (0.5 * em * (1.0 + ie)) / nrm;
   S = mk(-o0 * sh, o1 * sh);
   R = -o2 * sh;
 }
@@ -920,7 +925,7 @@ __host__ __device__ inline size_t window_lds_bytes(int n, int N, int wg, int K, 
 // End-of-window rows without LDS-resident columns: Gn[:, own] = Gc[:, own] + Yt W, with the
 // row of Yt (4 MB entries, zero past 4m) in registers and 4 own columns per step; m >= 1.
 template <int MB>
-__device__ inline void window_rows_glob(const cplx* __restrict__ Gc, cplx* __restrict__ Gn, const cplx* W,
+__device__ inline void window_rows_glob(const gcplx* __restrict__ Gc, gcplx* __restrict__ Gn, const cplx* W,
                                         const int* accs, int m, int n, int N, int cb0, int ncol, int ncs) {
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
     cplx y[4 * MB];
@@ -928,14 +933,14 @@ __device__ inline void window_rows_glob(const cplx* __restrict__ Gc, cplx* __res
     for (int t = 0; t < 4 * MB; ++t) {
       const int b = min(t >> 2, m - 1), k = t & 3;
       const int row = accs[b] + k * N;
-      cplx v = Gc[(size_t)row * n + r];
+      cplx v = ld(Gc + (size_t)row * n + r);
       if (r == row) v.re -= 1.0;
       y[t] = (t < 4 * m) ? v : mk(0, 0);
     }
     for (int j0 = 0; j0 < ncol; j0 += 4) {
       cplx acc[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = Gc[(size_t)(cb0 + min(j0 + q, ncol - 1)) * n + r];
+      for (int q = 0; q < 4; ++q) acc[q] = ld(Gc + (size_t)(cb0 + min(j0 + q, ncol - 1)) * n + r);
 #pragma unroll
       for (int t = 0; t < 4 * MB; ++t) {
         const cplx* w = W + (size_t)min(t, 4 * m - 1) * ncs;
@@ -945,9 +950,7 @@ __device__ inline void window_rows_glob(const cplx* __restrict__ Gc, cplx* __res
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if (j0 + q < ncol) {
-          cplx* d = Gn + (size_t)(cb0 + j0 + q) * n + r;
-          d->re = acc[q].re;
-          d->im = acc[q].im;
+          st(Gn + (size_t)(cb0 + j0 + q) * n + r, acc[q]);
         }
     }
   }
@@ -978,16 +981,16 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
   cplx* Ar = Bw + (size_t)16 * K;      // [K][16]    A_m(r_j, :)
   __shared__ LocalShared sh;
   __shared__ int accs[kWinMax];  // sites accepted in the current window, in order
-  cplx* X[2] = {c.mat[M_G], c.mat[M_GALT]};
+  gcplx* X[2] = {gp(c.mat[M_G]), gp(c.mat[M_GALT])};
   int R = (N + K - 1) / K;
   R += R & 1;
   if (TILE) {
-    const cplx* G0 = X[0] + (size_t)cb0 * n;
+    const gcplx* G0 = X[0] + (size_t)cb0 * n;
     const int tot = ncol * n;
     for (int e0 = tid; e0 < tot; e0 += 4 * blockDim.x) {
       cplx v[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = G0[min(e0 + q * (int)blockDim.x, tot - 1)];
+      for (int q = 0; q < 4; ++q) v[q] = ld(G0 + min(e0 + q * (int)blockDim.x, tot - 1));
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if (e0 + q * (int)blockDim.x < tot) {
@@ -1001,12 +1004,12 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
   int stop = 0;  // the uniform stream ran out: later windows make no proposals
   for (int rw = 0; rw < R; ++rw) {
     const int s0 = (int)((long long)rw * N / R), s1 = (int)((long long)(rw + 1) * N / R), kw = s1 - s0;
-    const cplx* Gc = X[rw & 1];
-    cplx* Gn = X[(rw + 1) & 1];
+    const gcplx* Gc = X[rw & 1];
+    gcplx* Gn = X[(rw + 1) & 1];
     // Gp[j][k](a, b) = G(r_j + a N, c_k + b N) for the window's sites j, k; U = 0, H = 0
     for (int e = tid; e < kw * kw * 16; e += blockDim.x) {
       const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
-      Gp[(jj * K + kk) * 16 + a * 4 + b] = Gc[(size_t)(s0 + kk + b * N) * n + (s0 + jj + a * N)];
+      Gp[(jj * K + kk) * 16 + a * 4 + b] = ld(Gc + (size_t)(s0 + kk + b * N) * n + (s0 + jj + a * N));
     }
     for (int e = tid; e < kw * K4 * 4; e += blockDim.x) Us[e] = mk(0, 0);
     for (int e = tid; e < kw * 4 * ncs; e += blockDim.x) Hs[e] = mk(0, 0);
@@ -1034,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
         cplx g4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          g4[q] = TILE ? T[(size_t)jl * n + site + q * N] : Gc[(size_t)(cb0 + jl) * n + site + q * N];
+          g4[q] = TILE ? T[(size_t)jl * n + site + q * N] : ld(Gc + (size_t)(cb0 + jl) * n + site + q * N);
           g4[q] += Hs[((size_t)ii * 4 + q) * ncs + jl];
         }
 #pragma unroll
@@ -1129,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
               const int row = accs[b] + k * N;
-              y[k] = Gc[(size_t)row * n + r];
+              y[k] = ld(Gc + (size_t)row * n + r);
               if (r == row) y[k].re -= 1.0;
             }
 #pragma unroll
@@ -1144,15 +1147,12 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
             if (j0 + q < ncol) {
               T[(size_t)(j0 + q) * n + r].re = acc[q].re;
               T[(size_t)(j0 + q) * n + r].im = acc[q].im;
-              cplx* d = Gn + (size_t)(cb0 + j0 + q) * n + r;
-              d->re = acc[q].re;
-              d->im = acc[q].im;
+              st(Gn + (size_t)(cb0 + j0 + q) * n + r, acc[q]);
             }
         }
     } else if (m == 0) {
       for (int e = tid; e < ncol * n; e += blockDim.x) {
-        const cplx v = Gc[(size_t)cb0 * n + e];
-        Gn[(size_t)cb0 * n + e] = v;
+        st(Gn + (size_t)cb0 * n + e, ld(Gc + (size_t)cb0 * n + e));
       }
     } else {
       switch ((m + 1) >> 1) {
