@@ -937,10 +937,17 @@ __device__ inline void window_rows_glob(const gcplx* __restrict__ Gc, gcplx* __r
       if (r == row) v.re -= 1.0;
       y[t] = (t < 4 * m) ? v : mk(0, 0);
     }
+    // the next four columns' loads are issued before this group's products (one group of
+    // HBM latency in flight while computing)
+    cplx nxt[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) nxt[q] = ld(Gc + (size_t)(cb0 + min(q, ncol - 1)) * n + r);
     for (int j0 = 0; j0 < ncol; j0 += 4) {
       cplx acc[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = ld(Gc + (size_t)(cb0 + min(j0 + q, ncol - 1)) * n + r);
+      for (int q = 0; q < 4; ++q) acc[q] = nxt[q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) nxt[q] = ld(Gc + (size_t)(cb0 + min(j0 + 4 + q, ncol - 1)) * n + r);
 #pragma unroll
       for (int t = 0; t < 4 * MB; ++t) {
         const cplx* w = W + (size_t)min(t, 4 * m - 1) * ncs;
