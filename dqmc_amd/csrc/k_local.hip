@@ -1134,14 +1134,21 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
           cplx acc[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) acc[q] = T[(size_t)min(j0 + q, ncol - 1) * n + r];
+          // Yt's row r for accepted site b + 1 is loaded while site b's products are formed
+          cplx yn[4];
+          const int a0 = m > 0 ? accs[0] : 0;  // (accs[0] is unset in a window without acceptances)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) yn[k] = ld(Gc + (size_t)(a0 + k * N) * n + r);
           for (int b = 0; b < m; ++b) {
             cplx y[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-              const int row = accs[b] + k * N;
-              y[k] = ld(Gc + (size_t)row * n + r);
-              if (r == row) y[k].re -= 1.0;
+              y[k] = yn[k];
+              if (r == accs[b] + k * N) y[k].re -= 1.0;
             }
+            const int bn = min(b + 1, m - 1);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) yn[k] = ld(Gc + (size_t)(accs[bn] + k * N) * n + r);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
               const cplx* wr = Wb + ((size_t)b * 4 + k) * ncs;
