@@ -76,6 +76,12 @@ __device__ inline void st(gcplx* p, cplx v) {
   p->re = v.re;
   p->im = v.im;
 }
+// streaming store: for data the next reader takes from another XCD (less for the next
+// agent-scope release to write back from this XCD's L2)
+__device__ inline void st_nt(gcplx* p, cplx v) {
+  __builtin_nontemporal_store(v.re, &p->re);
+  __builtin_nontemporal_store(v.im, &p->im);
+}
 
 // ---------------------------------------------------------------------------
 // Reductions (wave64)

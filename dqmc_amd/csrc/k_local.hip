@@ -957,7 +957,7 @@ __device__ inline void window_rows_glob(const gcplx* __restrict__ Gc, gcplx* __r
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if (j0 + q < ncol) {
-          st(Gn + (size_t)(cb0 + j0 + q) * n + r, acc[q]);
+          st_nt(Gn + (size_t)(cb0 + j0 + q) * n + r, acc[q]);
         }
     }
   }
@@ -1014,9 +1014,29 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
     const gcplx* Gc = X[rw & 1];
     gcplx* Gn = X[(rw + 1) & 1];
     // Gp[j][k](a, b) = G(r_j + a N, c_k + b N) for the window's sites j, k; U = 0, H = 0
-    for (int e = tid; e < kw * kw * 16; e += blockDim.x) {
-      const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
-      Gp[(jj * K + kk) * 16 + a * 4 + b] = ld(Gc + (size_t)(s0 + kk + b * N) * n + (s0 + jj + a * N));
+    // eight loads per thread in flight before the first store (one HBM latency per 2048
+    // entries); the variant without LDS-resident columns has no registers to spare for it
+    if (!TILE)
+      for (int e = tid; e < kw * kw * 16; e += blockDim.x) {
+        const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
+        Gp[(jj * K + kk) * 16 + a * 4 + b] = ld(Gc + (size_t)(s0 + kk + b * N) * n + (s0 + jj + a * N));
+      }
+    for (int e0 = 0; TILE && e0 < kw * kw * 16; e0 += 8 * 256) {
+      cplx v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = min(e0 + tid + 256 * j, kw * kw * 16 - 1);
+        const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
+        v[j] = ld(Gc + (size_t)(s0 + kk + b * N) * n + (s0 + jj + a * N));
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = e0 + tid + 256 * j;
+        if (e < kw * kw * 16) {
+          const int jk = e >> 4, jj = jk / kw, kk = jk - jj * kw, a = (e >> 2) & 3, b = e & 3;
+          Gp[(jj * K + kk) * 16 + a * 4 + b] = v[j];
+        }
+      }
     }
     for (int e = tid; e < kw * K4 * 4; e += blockDim.x) Us[e] = mk(0, 0);
     for (int e = tid; e < kw * 4 * ncs; e += blockDim.x) Hs[e] = mk(0, 0);
@@ -1161,12 +1181,12 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
             if (j0 + q < ncol) {
               T[(size_t)(j0 + q) * n + r].re = acc[q].re;
               T[(size_t)(j0 + q) * n + r].im = acc[q].im;
-              st(Gn + (size_t)(cb0 + j0 + q) * n + r, acc[q]);
+              st_nt(Gn + (size_t)(cb0 + j0 + q) * n + r, acc[q]);
             }
         }
     } else if (m == 0) {
       for (int e = tid; e < ncol * n; e += blockDim.x) {
-        st(Gn + (size_t)cb0 * n + e, ld(Gc + (size_t)cb0 * n + e));
+        st_nt(Gn + (size_t)cb0 * n + e, ld(Gc + (size_t)cb0 * n + e));
       }
     } else {
       switch ((m + 1) >> 1) {
