@@ -187,6 +187,23 @@ __device__ inline void wave_reduce_scatter(double (&v)[V]) {
   row_stage<0xB1, 0>(w1, lane);  // quad_perm [1,0,3,2]: l ^ 1
 }
 
+// wave_sum of each of NB complex values (NB = 4, 8, 16 or 32): one reduce-scatter, then the
+// totals are read back into every lane (scalar registers)
+template <int NB>
+__device__ inline void wave_sum_many(cplx (&w)[NB]) {
+  constexpr int V = 2 * NB;
+  constexpr int LS = V == 8 ? 3 : V == 16 ? 2 : V == 32 ? 1 : 0;  // lane l holds value l >> LS
+  double v[V];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    v[2 * j] = w[j].re;
+    v[2 * j + 1] = w[j].im;
+  }
+  wave_reduce_scatter<V>(v);
+#pragma unroll
+  for (int j = 0; j < NB; ++j) w[j] = mk(read_lane(v[0], (2 * j) << LS), read_lane(v[0], (2 * j + 1) << LS));
+}
+
 // XCD-aware workgroup numbering. The dispatcher hands consecutive workgroups of a launch
 // to the 8 XCDs in turn, and each XCD has its own L2. Renumbering so that every XCD gets a
 // contiguous range of logical blocks puts the workgroups of one chain (blockIdx.y) on as

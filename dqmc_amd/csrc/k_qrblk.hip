@@ -25,7 +25,6 @@ namespace dqmc {
 constexpr int kQrMaxN = 1024;
 constexpr int kQrMaxNb = 16;
 
-__device__ inline cplx wsum(cplx v) { return wave_sum(v); }
 __device__ inline cplx shfl_c(cplx v, int src) { return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64)); }
 __device__ inline cplx read_lane_c(cplx v, int lane) { return mk(read_lane(v.re, lane), read_lane(v.im, lane)); }
 
@@ -488,8 +487,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
 #pragma unroll
       for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], ak);
     }
-#pragma unroll
-    for (int j = 0; j < NB; ++j) wv[j] = wsum(wv[j]);
+    wave_sum_many<NB>(wv);
     if (RS > 1) {
       if (lane == 0) {
 #pragma unroll
@@ -630,8 +628,7 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
 #pragma unroll
       for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], ak);
     }
-#pragma unroll
-    for (int j = 0; j < NB; ++j) wv[j] = wsum(wv[j]);
+    wave_sum_many<NB>(wv);
     if (RS > 1) {
       if (lane == 0) {
 #pragma unroll
