@@ -219,6 +219,45 @@ def _greens_kat_case(mod, DQMC, route, panel):
     c.close()
 
 
+def _oracle_greens(Ul, Dl, Tl, Ur, Dr, Tr):
+    """G and log|det G| by the oracle's restatement of calculate_greens and calculate_logdet
+    (stack.jl:338-385: column-pivoted UDTs, myrdiv, inv), on a bare stack object."""
+    m = O.DQMC.__new__(O.DQMC)
+    m.Ul, m.Dl, m.Tl, m.Ur, m.Dr, m.Tr = Ul, Dl, Tl, Ur, Dr, Tr
+    m.calculate_greens()
+    return m.greens, m.calculate_logdet()
+
+
+@pytest.mark.parametrize("L", [12, 16])
+def test_calculate_greens_large_n(amd, L):
+    """calculate_greens at n = 576 and 1024, where the blocked QR keeps four rows per thread
+    and the QR solve takes one column per workgroup: both device routes against the oracle on
+    random UDT factors whose D spans 1e-8 .. 1e8 on each side (I + Bl Br^H then has a
+    condition number far beyond a direct inverse's reach, which is what the UDT route is for)."""
+    mod, DQMC = amd
+    n = 4 * L * L
+    _say("calculate_greens n=%d, both routes, against the oracle" % n)
+    rng = np.random.default_rng(40 + L)
+
+    def udt():
+        U = np.linalg.qr(rand_c(rng, n))[0]
+        D = np.sort(10.0 ** rng.uniform(-8, 8, n))[::-1].copy()
+        T = np.eye(n) + 0.1 * np.triu(rand_c(rng, n), 1)
+        return U, D, T
+
+    Ul, Dl, Tl = udt()
+    Ur, Dr, Tr = udt()
+    G_ref, ld_ref = _oracle_greens(Ul, Dl, Tl, Ur, Dr, Tr)
+    c = DQMC(config_params(mod, L, 10))
+    for route in (0, 1):
+        c.set_greens_route(route)
+        G, ldet = c.calculate_greens(Ul, Dl, Tl, Ur, Dr, Tr)
+        err = np.abs(G - G_ref).max() / np.abs(G_ref).max()
+        assert err < 1e-10, (route, err)
+        assert abs(ldet - ld_ref) < 1e-10 * abs(ld_ref), (route, ldet, ld_ref)
+    c.close()
+
+
 # --------------------------------------------------------------------------- chain
 @pytest.mark.parametrize("route,panel", [(0, 8), (0, 16), (1, 8)])
 def test_propagation_full_cycle_fixture(amd, route, panel):
