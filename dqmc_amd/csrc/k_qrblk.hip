@@ -132,10 +132,11 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
 
   // ---- factor the panel at logical column k0 (panel index p) held in x; rows >= kw of x are
   // valid and written back. Per column t (zlarfg, then H^H on the columns after it):
-  //   one butterfly sums |x_t|^2 and x_t^H a_u over the rows below the pivot, plus the zlarft
-  //   products S(j, t-1) = v_j^H v_{t-1}; the pivot row goes through LDS; after one barrier
-  //   every thread forms beta, tau and 1/(alpha - beta) from the same sums and updates its rows:
-  //   v^H a_u = a_u(i) + conj(1/(alpha - beta)) x_t^H a_u.
+  //   one butterfly sums |x_t|^2 and x_t^H a_u (u > t) over the rows below the pivot, plus
+  //   v_j^H x_t (j < t) for zlarft; the pivot row goes through LDS; after one barrier every
+  //   thread forms beta, tau and 1/(alpha - beta) from the same sums and updates its rows:
+  //   v^H a_u = a_u(i) + conj(1/(alpha - beta)) x_t^H a_u, and wave 0 stores
+  //   S(j, t) = v_j^H v_t = conj(v_j(i)) + (1/(alpha - beta)) v_j^H x_t.
   // Every call site of this and of the block update below is single, and the kernel's code
   // fits the instruction cache: the panel loop runs without instruction fetch stalls.
   auto factor_regs = [&](int k0, int p, int kw) __attribute__((always_inline)) {
@@ -158,21 +159,18 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
           v[e] = fma(xt.re, x[q][u].re, fma(xt.im, x[q][u].im, v[e]));
           v[e + 1] = fma(xt.re, x[q][u].im, fma(-xt.im, x[q][u].re, v[e + 1]));
         }
-        if constexpr (t >= 2) {  // S(j, t-1), j < t-1: rows >= i-1, v_{t-1} has its unit at row i-1
-          const cplx b = (r == i - 1) ? mk(1, 0) : (r > i - 1) ? x[q][t - 1] : mk(0, 0);
 #pragma unroll
-          for (int j = 0; j < t - 1; ++j) {
-            const int e = 1 + 2 * (NB - 1 - t) + 2 * j;
-            v[e] = fma(x[q][j].re, b.re, fma(x[q][j].im, b.im, v[e]));
-            v[e + 1] = fma(x[q][j].re, b.im, fma(-x[q][j].im, b.re, v[e + 1]));
-          }
+        for (int j = 0; j < t; ++j) {  // v_j^H x_t below the pivot, for S(j, t)
+          const int e = 1 + 2 * (NB - 1 - t) + 2 * j;
+          v[e] = fma(x[q][j].re, xt.re, fma(x[q][j].im, xt.im, v[e]));
+          v[e + 1] = fma(x[q][j].re, xt.im, fma(-x[q][j].im, xt.re, v[e + 1]));
         }
       }
 #pragma unroll
       for (int q = 0; q < RQ; ++q)
-        if (tid + 256 * q == i) {  // the owner of the pivot row
+        if (tid + 256 * q == i) {  // the owner of the pivot row (all columns: v_j(i) for j < t too)
 #pragma unroll
-          for (int u = t; u < NB; ++u) {
+          for (int u = 0; u < NB; ++u) {
             piv[par][u].re = x[q][u].re;
             piv[par][u].im = x[q][u].im;
           }
@@ -221,42 +219,19 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
         for (int u = t + 1; u < NB; ++u) x[q][u] = x[q][u] - f[u] * vt;
         x[q][t] = (r > i) ? vt : (r == i) ? mk(beta, 0) : xt;
       }
-      if (t >= 2 && wave == 0 && lane < t - 1) {  // lane j: S(j, t-1)
+      if (wave == 0 && lane < t) {  // lane j: S(j, t) = v_j^H v_t = conj(v_j(i)) + sc v_j^H x_t
         const int e = 1 + 2 * (NB - 1 - t) + 2 * lane;
-        Tl[lane * NB + t - 1] = mk((red[par][0][e] + red[par][1][e]) + (red[par][2][e] + red[par][3][e]),
-                                   (red[par][0][e + 1] + red[par][1][e + 1]) + (red[par][2][e + 1] + red[par][3][e + 1]));
+        const cplx raw = mk((red[par][0][e] + red[par][1][e]) + (red[par][2][e] + red[par][3][e]),
+                            (red[par][0][e + 1] + red[par][1][e + 1]) + (red[par][2][e + 1] + red[par][3][e + 1]));
+        cplx sj = conj(mk(piv[par][lane].re, piv[par][lane].im));
+        cfma(sj, sc, raw);
+        Tl[lane * NB + t] = sj;
       }
       if (tid == 0) {
         taul[t] = tau;
         betal[t] = beta;
       }
     });
-    {  // S(j, NB-1), j < NB-1
-      constexpr int par = NB & 1;
-      double v[V];
-#pragma unroll
-      for (int e = 0; e < V; ++e) v[e] = 0.0;
-      const int rp = k0 + NB - 1;
-#pragma unroll
-      for (int q = 0; q < RQ; ++q) {
-        const int r = tid + 256 * q;
-        const cplx b = (r == rp) ? mk(1, 0) : (r > rp) ? x[q][NB - 1] : mk(0, 0);
-#pragma unroll
-        for (int j = 0; j < NB - 1; ++j) {
-          v[2 * j] = fma(x[q][j].re, b.re, fma(x[q][j].im, b.im, v[2 * j]));
-          v[2 * j + 1] = fma(x[q][j].re, b.im, fma(-x[q][j].im, b.re, v[2 * j + 1]));
-        }
-      }
-      wave_reduce_scatter<V>(v);
-      if ((lane & ((1 << LS) - 1)) == 0) red[par][wave][lane >> LS] = v[0];
-      __syncthreads();
-      if (wave == 0 && lane < NB - 1) {
-        const int e = 2 * lane;
-        Tl[lane * NB + NB - 1] = mk((red[par][0][e] + red[par][1][e]) + (red[par][2][e] + red[par][3][e]),
-                                    (red[par][0][e + 1] + red[par][1][e + 1]) + (red[par][2][e + 1] + red[par][3][e + 1]));
-      }
-      __syncthreads();
-    }
     if (tid < kb) {
       D[k0 + tid] = fabs(betal[tid]);
       st(gp(c.qr_tau) + k0 + tid, taul[tid]);
