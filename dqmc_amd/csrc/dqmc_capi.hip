@@ -80,7 +80,7 @@ struct Group {
   int local_variant = 0;  // 0 auto, 1 G in global memory, 2 LDS-resident G tiles, 3 delayed updates,
                           // 4 windowed delayed updates
   int udt_route = 0;      // 0: blocked QR with pre-pivoting, 1: column-pivoted QR (zgeqp3 semantics)
-  int qr_nb = 0;          // panel width of the blocked QR (0: 16 up to n = 512, else 8)
+  int qr_nb = 0;          // panel width of the blocked QR (0: the default, 8)
   bool recomputed = false;
   // one sweep (M update steps) captured as a HIP graph per starting state and settings
   struct SweepGraph {
@@ -332,9 +332,9 @@ int default_groups(Group& g) {
   // variants were tuned at when the window does not fit
   g.wg_local = std::max(1, std::min({per, g.n / 4, kMaxWG}));
   if (local_window_sites(g.n, g.wg_local) < 2) g.wg_local = std::max(1, std::min({per, g.n / 8, kMaxWG}));
-  // a lone chain's QR is latency-bound: the narrower panel shortens each panel's serial part
-  // (the autotuner measures this per shape); many chains share the chip better with 16
-  g.qr_nb = nch <= 8 ? 8 : 0;
+  // 8-column panels: each panel's serial part is shorter and the panel stays in registers
+  // (the autotuner measures 16 too)
+  g.qr_nb = 8;
   return 0;
 }
 
@@ -1213,7 +1213,7 @@ int dqmc_batch_set_qr_panel(dqmc_batch* b, int nb) {
   RC(batch_sync(b));
   for (auto& g : b->groups) {
     g.drop_graphs();
-    g.qr_nb = nb ? nb : (g.nch() <= 8 ? 8 : 0);  // the default of default_groups
+    g.qr_nb = nb ? nb : 8;  // the default of default_groups
   }
   return 0;
 }

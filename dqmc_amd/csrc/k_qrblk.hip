@@ -653,7 +653,12 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
   }
 }
 
-int qr_panel_width(int nmax) { return nmax <= 512 ? 16 : 8; }
+// default panel width: 8 (the register-resident panel keeps nb x rows-per-thread values per
+// thread; at nb = 16 the per-column chain is longer and the kernel spills)
+int qr_panel_width(int nmax) {
+  (void)nmax;
+  return 8;
+}
 
 template <int NB, int RQ>
 hipError_t launch_qr_blocked_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,

@@ -268,8 +268,8 @@ int dqmc_autotune(dqmc_ctx* ctx, double box, int reps, double* report);
 int dqmc_batch_set_groups(dqmc_batch* b, int wg_qr, int wg_inv, int wg_local);
 int dqmc_batch_set_local_variant(dqmc_batch* b, int variant);
 /* panel width of the blocked QR (the stack's factorisations and calculate_greens): 16
- * (n <= 512) or 8; 0 restores the default (8 for a context or a batch of a few chains, 16
- * otherwise at n <= 512). No reference counterpart; results agree to rounding. */
+ * (n <= 512) or 8; 0 restores the default, 8. No reference counterpart; results agree to
+ * rounding. */
 int dqmc_set_qr_panel(dqmc_ctx* ctx, int nb);
 int dqmc_batch_set_qr_panel(dqmc_batch* b, int nb);
 
