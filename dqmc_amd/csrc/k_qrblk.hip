@@ -242,15 +242,16 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
       cplx trow[NB];
 #pragma unroll
       for (int l = 0; l < NB; ++l) trow[l] = mk(0, 0);
+      // no branch on t < kb around the LDS reads (that would wait for each read in turn):
+      // the S entries and tau of the zero columns past kb are zero, and a select keeps them 0
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        if (t < kb) {
-          cplx acc = mk(0, 0);
+        cplx acc = mk(0, 0);
 #pragma unroll
-          for (int l = 0; l < t; ++l) cfma(acc, trow[l], Tl[l * NB + t]);
-          const cplx tt = taul[t];
-          trow[t] = (lane < t) ? -(tt * acc) : (lane == t) ? tt : mk(0, 0);
-        }
+        for (int l = 0; l < t; ++l) cfma(acc, trow[l], Tl[l * NB + t]);
+        const cplx tt = taul[t];
+        const cplx v = (lane < t) ? -(tt * acc) : (lane == t) ? tt : mk(0, 0);
+        trow[t] = (t < kb) ? v : mk(0, 0);
       }
       if (lane < NB) {
 #pragma unroll
