@@ -26,16 +26,27 @@ struct StageList {
   int count;
 };
 
-// y = op(P) x for a 4x4 row-major P
-__device__ inline void mix4(const cplx* P, int mode, cplx x[4], cplx y[4]) {
+// eV site block (interactions.jl:102-140): [[C,S,0,R],[S*,C,-R,0],[0,-R,C,S*],[R,0,S,C]]
+__device__ inline void ev_block(double C, cplx S, double R, cplx E[16]) {
+  const cplx z = mk(0, 0);
+  E[0] = mk(C, 0);   E[1] = S;          E[2] = z;          E[3] = mk(R, 0);
+  E[4] = conj(S);    E[5] = mk(C, 0);   E[6] = mk(-R, 0);  E[7] = z;
+  E[8] = z;          E[9] = mk(-R, 0);  E[10] = mk(C, 0);  E[11] = conj(S);
+  E[12] = mk(R, 0);  E[13] = z;         E[14] = S;         E[15] = mk(C, 0);
+}
+
+// y = op(P) x for a 4x4 row-major P; the mode is a template argument so that the loops that
+// call it hold no branch (stages dispatch on the mode once)
+template <int MODE>
+__device__ inline void mix4(const cplx* P, cplx x[4], cplx y[4]) {
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     cplx acc = mk(0, 0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (mode == MODE_N)
+      if (MODE == MODE_N)
         cfma(acc, P[a * 4 + k], x[k]);
-      else if (mode == MODE_T)
+      else if (MODE == MODE_T)
         cfma(acc, P[k * 4 + a], x[k]);
       else
         cfma_cj(acc, P[k * 4 + a], x[k]);
@@ -44,13 +55,40 @@ __device__ inline void mix4(const cplx* P, int mode, cplx x[4], cplx y[4]) {
   }
 }
 
-// eV site block (interactions.jl:102-140): [[C,S,0,R],[S*,C,-R,0],[0,-R,C,S*],[R,0,S,C]]
-__device__ inline void ev_block(double C, cplx S, double R, cplx E[16]) {
-  const cplx z = mk(0, 0);
-  E[0] = mk(C, 0);   E[1] = S;          E[2] = z;          E[3] = mk(R, 0);
-  E[4] = conj(S);    E[5] = mk(C, 0);   E[6] = mk(-R, 0);  E[7] = z;
-  E[8] = z;          E[9] = mk(-R, 0);  E[10] = mk(C, 0);  E[11] = conj(S);
-  E[12] = mk(R, 0);  E[13] = z;         E[14] = S;         E[15] = mk(C, 0);
+// the eV stage and a plaquette stage on the tile, for one mode
+template <int MODE>
+__device__ inline void ev_stage(cplx* vec, const double* evc, double power, int n, int N, int nv) {
+  for (int idx = threadIdx.x; idx < nv * N; idx += blockDim.x) {
+    const int v = idx / N, i = idx - v * N;
+    cplx E[16];
+    ev_block(evc[4 * i], mk(power * evc[4 * i + 1], power * evc[4 * i + 2]), power * evc[4 * i + 3], E);
+    cplx x[4], y[4];
+    cplx* base = vec + (size_t)v * n + i;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[a] = base[a * N];
+    mix4<MODE>(E, x, y);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) base[a * N] = y[a];
+  }
+}
+template <int MODE>
+__device__ inline void plaq_stage(cplx* vec, const cplx* P0, const cplx* Puni, const int* sites, int n, int N,
+                                  int nv) {
+  const int nplaq = N / 4;
+  for (int idx = threadIdx.x; idx < nv * 4 * nplaq; idx += blockDim.x) {
+    const int v = idx / (4 * nplaq);
+    const int rem = idx - v * 4 * nplaq;
+    const int b = rem / nplaq, pq = rem - b * nplaq;
+    const cplx* P = Puni ? Puni + b * 16 : P0 + ((size_t)b * nplaq + pq) * 16;
+    const int* st4 = sites + 4 * pq;
+    cplx* base = vec + (size_t)v * n + b * N;
+    cplx x[4], y[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[a] = base[st4[a]];
+    mix4<MODE>(P, x, y);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) base[st4[a]] = y[a];
+  }
 }
 
 // C, S, R per site for power +1 (interactions.jl:52-58); the power sign is applied later.
@@ -128,36 +166,16 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
         vec[v * n + k] = mu[k] * vec[v * n + k];
       }
     } else if (st.kind == ST_EV) {
-      for (int idx = threadIdx.x; idx < nv * N; idx += blockDim.x) {
-        const int v = idx / N, i = idx - v * N;
-        cplx E[16];
-        const double p = st.power;
-        ev_block(evc[4 * i], mk(p * evc[4 * i + 1], p * evc[4 * i + 2]), p * evc[4 * i + 3], E);
-        cplx x[4], y[4];
-        cplx* base = vec + (size_t)v * n + i;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) x[a] = base[a * N];
-        mix4(E, st.mode, x, y);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) base[a * N] = y[a];
-      }
+      if (st.mode == MODE_N) ev_stage<MODE_N>(vec, evc, st.power, n, N, nv);
+      else if (st.mode == MODE_T) ev_stage<MODE_T>(vec, evc, st.power, n, N, nv);
+      else ev_stage<MODE_C>(vec, evc, st.power, n, N, nv);
     } else {  // ST_PLAQ
       const cplx* P0 = c.plaq + (((size_t)st.var * 2 + st.group) * 4) * nplaq * 16;
       const int* sites = c.plaq_sites + (size_t)st.group * nplaq * 4;
-      for (int idx = threadIdx.x; idx < nv * 4 * nplaq; idx += blockDim.x) {
-        const int v = idx / (4 * nplaq);
-        const int rem = idx - v * 4 * nplaq;
-        const int b = rem / nplaq, pq = rem - b * nplaq;
-        const cplx* P = uni ? pl_lds[s][b] : P0 + ((size_t)b * nplaq + pq) * 16;
-        const int* st4 = sites + 4 * pq;
-        cplx* base = vec + (size_t)v * n + b * N;
-        cplx x[4], y[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) x[a] = base[st4[a]];
-        mix4(P, st.mode, x, y);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) base[st4[a]] = y[a];
-      }
+      const cplx* Puni = uni ? &pl_lds[s][0][0] : nullptr;
+      if (st.mode == MODE_N) plaq_stage<MODE_N>(vec, P0, Puni, sites, n, N, nv);
+      else if (st.mode == MODE_T) plaq_stage<MODE_T>(vec, P0, Puni, sites, n, N, nv);
+      else plaq_stage<MODE_C>(vec, P0, Puni, sites, n, N, nv);
     }
     __syncthreads();
   }
