@@ -71,7 +71,10 @@ __device__ inline void ev_stage(cplx* vec, const double* evc, double power, int 
     for (int a = 0; a < 4; ++a) base[a * N] = y[a];
   }
 }
-template <int MODE>
+// UNI: every plaquette of a block has the same factor, staged in LDS (Puni); otherwise each
+// plaquette's factor is read from global memory (kept apart: a pointer that may point to
+// either compiles to FLAT loads)
+template <int MODE, bool UNI>
 __device__ inline void plaq_stage(cplx* vec, const cplx* P0, const cplx* Puni, const int* sites, int n, int N,
                                   int nv) {
   const int nplaq = N / 4;
@@ -79,8 +82,16 @@ __device__ inline void plaq_stage(cplx* vec, const cplx* P0, const cplx* Puni, c
     const int v = idx / (4 * nplaq);
     const int rem = idx - v * 4 * nplaq;
     const int b = rem / nplaq, pq = rem - b * nplaq;
-    const cplx* P = Puni ? Puni + b * 16 : P0 + ((size_t)b * nplaq + pq) * 16;
-    const int* st4 = sites + 4 * pq;
+    cplx P[16];
+    if (UNI) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) P[e] = Puni[b * 16 + e];
+    } else {
+      const gcplx* Pg = gp(P0) + ((size_t)b * nplaq + pq) * 16;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) P[e] = ld(Pg + e);
+    }
+    const gint* st4 = gp(sites) + 4 * pq;
     cplx* base = vec + (size_t)v * n + b * N;
     cplx x[4], y[4];
 #pragma unroll
@@ -146,12 +157,12 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
   if (!ROWS) {
     for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
       const int v = idx / n, k = idx - v * n;
-      vec[v * n + k] = X[(size_t)(v0 + v) * n + k];
+      vec[v * n + k] = ld(gp(X) + (size_t)(v0 + v) * n + k);
     }
   } else {
     for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
       const int k = idx / nv, v = idx - k * nv;
-      vec[v * n + k] = X[(size_t)k * n + (v0 + v)];
+      vec[v * n + k] = ld(gp(X) + (size_t)k * n + (v0 + v));
     }
   }
   __syncthreads();
@@ -172,10 +183,16 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
     } else {  // ST_PLAQ
       const cplx* P0 = c.plaq + (((size_t)st.var * 2 + st.group) * 4) * nplaq * 16;
       const int* sites = c.plaq_sites + (size_t)st.group * nplaq * 4;
-      const cplx* Puni = uni ? &pl_lds[s][0][0] : nullptr;
-      if (st.mode == MODE_N) plaq_stage<MODE_N>(vec, P0, Puni, sites, n, N, nv);
-      else if (st.mode == MODE_T) plaq_stage<MODE_T>(vec, P0, Puni, sites, n, N, nv);
-      else plaq_stage<MODE_C>(vec, P0, Puni, sites, n, N, nv);
+      const cplx* Puni = &pl_lds[s][0][0];
+      if (uni) {
+        if (st.mode == MODE_N) plaq_stage<MODE_N, true>(vec, P0, Puni, sites, n, N, nv);
+        else if (st.mode == MODE_T) plaq_stage<MODE_T, true>(vec, P0, Puni, sites, n, N, nv);
+        else plaq_stage<MODE_C, true>(vec, P0, Puni, sites, n, N, nv);
+      } else {
+        if (st.mode == MODE_N) plaq_stage<MODE_N, false>(vec, P0, Puni, sites, n, N, nv);
+        else if (st.mode == MODE_T) plaq_stage<MODE_T, false>(vec, P0, Puni, sites, n, N, nv);
+        else plaq_stage<MODE_C, false>(vec, P0, Puni, sites, n, N, nv);
+      }
     }
     __syncthreads();
   }
@@ -183,12 +200,12 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
   if (!ROWS) {
     for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
       const int v = idx / n, k = idx - v * n;
-      X[(size_t)(v0 + v) * n + k] = vec[v * n + k];
+      st(gp(X) + (size_t)(v0 + v) * n + k, vec[v * n + k]);
     }
   } else {
     for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
       const int k = idx / nv, v = idx - k * nv;
-      X[(size_t)k * n + (v0 + v)] = vec[v * n + k];
+      st(gp(X) + (size_t)k * n + (v0 + v), vec[v * n + k]);
     }
   }
 }
