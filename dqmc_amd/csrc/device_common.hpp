@@ -5,9 +5,30 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 namespace dqmc {
+
+// Raises a kernel's dynamic-LDS limit to at least `lds` bytes on the current device. The
+// attribute is per device and per function; the cache of what has been set is shared by
+// every context and thread of the process, so it is guarded by a mutex.
+inline hipError_t ensure_dynamic_lds(const void* fn, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> set;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& cur = set[std::make_pair(dev, fn)];
+  if (lds <= cur) return hipSuccess;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) cur = lds;
+  return e;
+}
 
 struct cplx {
   double re, im;

@@ -322,9 +322,12 @@ int upload_desc(dqmc_batch* b, Group& g) {
   return 0;
 }
 
-int default_groups(Group& g) {
-  const int nch = g.nch();
-  int per = std::max(1, 256 / nch);
+// Workgroups per chain. The cooperative kernels of different shape groups run concurrently
+// on their own streams, and a cooperative launch waits for all of its workgroups to be
+// resident, so the budget of 256 (one workgroup per CU) is shared by every chain of the
+// batch, not only by the chains of this group.
+int default_groups(Group& g, int batch_chains) {
+  int per = std::max(1, 256 / std::max(1, batch_chains));
   g.wg_qr = std::max(1, std::min({per, g.n / 8, kMaxWG}));
   g.wg_inv = std::max(std::max(1, (g.n + 511) / 512), std::min({per, g.n / 8, kMaxWG}));
   // local updates: the windowed variant is fastest with many narrow column blocks (the
@@ -940,7 +943,7 @@ int batch_create(const dqmc_params* p, const dqmc_operators* ops, int nchains, i
   }
   for (auto& g : b->groups) {
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-    default_groups(g);
+    default_groups(g, nchains);
     int rc = upload_desc(b, g);
     if (rc) return rc;
   }
@@ -951,9 +954,19 @@ int batch_create(const dqmc_params* p, const dqmc_operators* ops, int nchains, i
 
 int batch_destroy(dqmc_batch* b) {
   if (!b) return 0;
-  (void)hipDeviceSynchronize();
+  (void)hipSetDevice(b->device);
+  for (auto& g : b->groups)
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
   for (auto& g : b->groups) {
     g.drop_graphs();
+    // timing events: pending records and the reuse pool
+    for (auto& r : g.prof) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    g.prof.clear();
+    for (hipEvent_t e : g.event_pool) (void)hipEventDestroy(e);
+    g.event_pool.clear();
     if (g.stream) (void)hipStreamDestroy(g.stream);
     if (g.d_desc) (void)hipFree(g.d_desc);
   }
@@ -1994,7 +2007,12 @@ int autotune_group(dqmc_batch* b, Group& g, double box, int reps, double* rep) {
     }
     return hipSuccess;
   };
-  const int cap = std::max(1, std::min({kMaxWG, std::max(1, 1024 / nch), g.n / 2}));
+  // a lone shape group may use several workgroups per CU (the launch fails when they do not
+  // fit, and that candidate is skipped); the groups of a mixed batch run concurrently, so
+  // their cooperative grids together must fit one workgroup per CU
+  const int total = (int)b->chains.size();
+  const int cap = b->groups.size() == 1 ? std::max(1, std::min({kMaxWG, std::max(1, 1024 / nch), g.n / 2}))
+                                        : std::max(1, std::min({kMaxWG, 256 / total, g.n / 2}));
   std::vector<int> wgs;
   for (int w : {1, 2, 3, 4, 6, 8, 12, 16, 25, 32, 50, 64, 100, 128})
     if (w <= cap) wgs.push_back(w);

@@ -227,15 +227,10 @@ static hipError_t run_stages(hipStream_t s, const ChainDesc* chains, int nchains
   while (vpb > 1 && (size_t)vpb * nmax * 16 > 65536) vpb >>= 1;
   const size_t lds = (size_t)vpb * nmax * 16 + (size_t)Nmax * 32;
   dim3 grid((nmax + vpb - 1) / vpb, nchains);
-  static size_t attr = 0;
-  if (lds > 65536 && lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_apply_stages<true>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)k_apply_stages<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+  {
+    hipError_t e = ensure_dynamic_lds(rows ? (const void*)k_apply_stages<true> : (const void*)k_apply_stages<false>,
+                                      lds);
     if (e != hipSuccess) return e;
-    attr = lds;
   }
   if (rows)
     hipLaunchKernelGGL(k_apply_stages<true>, grid, dim3(256), lds, s, chains, X, sl, slice, slices, vpb);

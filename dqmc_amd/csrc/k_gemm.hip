@@ -67,48 +67,6 @@ __device__ inline void load_tiles(const cplx* A, const cplx* B, int n, int i0, i
   }
 }
 
-// Vector-FMA variant (selected with DQMC_GEMM_VECTOR=1): each thread owns 4 rows of one column.
-template <int OPA, int OPB>
-__global__ __launch_bounds__(256) void k_gemm_vec(const ChainDesc* chains, MatRef Cr, MatRef Ar, MatRef Br,
-                                                  VecRef dlr, VecRef drr, double beta) {
-  const ChainDesc& c = chains[blockIdx.z];
-  const int n = c.n;
-  const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
-  if (i0 >= n || j0 >= n) return;
-  const cplx* A = matptr(c, Ar);
-  const cplx* B = matptr(c, Br);
-  cplx* C = matptr(c, Cr);
-  const double* dl = dlr.kind >= 0 ? vecptr(c, dlr) : nullptr;
-  const double* dr = drr.kind >= 0 ? vecptr(c, drr) : nullptr;
-  __shared__ double As_re[GT][KT + 1], As_im[GT][KT + 1];
-  __shared__ double Bs_re[KT][GT + 1], Bs_im[KT][GT + 1];
-  const int cc = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
-  cplx acc[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
-  for (int k0 = 0; k0 < n; k0 += KT) {
-    load_tiles<OPA, OPB>(A, B, n, i0, j0, k0, As_re, As_im, Bs_re, Bs_im);
-    __syncthreads();
-    for (int kk = 0; kk < KT; ++kk) {
-      const cplx b = mk(Bs_re[kk][cc], Bs_im[kk][cc]);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) cfma(acc[q], mk(As_re[r0 + q][kk], As_im[r0 + q][kk]), b);
-    }
-    __syncthreads();
-  }
-  const int col = j0 + cc;
-  for (int q = 0; q < 4; ++q) {
-    const int row = i0 + r0 + q;
-    if (row < n && col < n) {
-      double s = 1.0;
-      if (dl) s *= dl[row];
-      if (dr) s *= dr[col];
-      cplx val = s * acc[q];
-      const size_t o = (size_t)col * n + row;
-      if (beta != 0.0) val = val + beta * C[o];
-      C[o] = val;
-    }
-  }
-}
-
 template <int OPA, int OPB>
 __global__ __launch_bounds__(256) void k_gemm(const ChainDesc* chains, MatRef Cr, MatRef Ar, MatRef Br, VecRef dlr,
                                               VecRef drr, double beta) {
@@ -206,29 +164,9 @@ __global__ __launch_bounds__(256) void k_gemm(const ChainDesc* chains, MatRef Cr
   }
 }
 
-static int use_vector_gemm() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("DQMC_GEMM_VECTOR");
-    v = (e && e[0] == '1') ? 1 : 0;
-  }
-  return v;
-}
-
 hipError_t launch_gemm(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef C, MatRef A,
                        GemmOp opA, MatRef B, GemmOp opB, VecRef dl, VecRef dr, double beta) {
   dim3 grid((nmax + GT - 1) / GT, (nmax + GT - 1) / GT, nchains);
-  if (use_vector_gemm()) {
-    if (opA == OP_N && opB == OP_N)
-      hipLaunchKernelGGL((k_gemm_vec<OP_N, OP_N>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
-    else if (opA == OP_N && opB == OP_C)
-      hipLaunchKernelGGL((k_gemm_vec<OP_N, OP_C>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
-    else if (opA == OP_C && opB == OP_N)
-      hipLaunchKernelGGL((k_gemm_vec<OP_C, OP_N>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
-    else
-      hipLaunchKernelGGL((k_gemm_vec<OP_C, OP_C>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
-    return hipGetLastError();
-  }
   if (opA == OP_N && opB == OP_N)
     hipLaunchKernelGGL((k_gemm<OP_N, OP_N>), grid, dim3(256), 0, s, chains, C, A, B, dl, dr, beta);
   else if (opA == OP_N && opB == OP_C)

@@ -1238,7 +1238,6 @@ hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int ncha
   bool tile = false;
   int K = variant == 4 ? local_window_sites(nmax, wg, &tile) : 0;
   if (variant == 4 && K < 2) return hipErrorInvalidValue;
-  const int slot = variant == 4 && tile ? 5 : variant;
   const void* fn = variant == 2   ? (const void*)k_local_updates_tile
                    : variant == 3 ? (const void*)k_local_updates_delayed
                    : variant == 4 ? (tile ? (const void*)k_local_updates_window<true>
@@ -1249,11 +1248,9 @@ hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int ncha
                      : variant == 4 ? window_lds_bytes(nmax, Nmax, wg, K, tile)
                                     : local_lds_global(nmax, Nmax, wg);
   if (lds > (variant == 4 ? kWindowLdsMax : kLocalLdsMax)) return hipErrorInvalidValue;
-  static size_t attr[6] = {0, 0, 0, 0, 0, 0};
-  if (lds > 65536 && lds > attr[slot]) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  {
+    hipError_t e = ensure_dynamic_lds(fn, lds);
     if (e != hipSuccess) return e;
-    attr[slot] = lds;
   }
   void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log, (void*)&K};
   hipError_t e = hipLaunchCooperativeKernel(fn, dim3(wg, nchains), dim3(256), args, lds, s);

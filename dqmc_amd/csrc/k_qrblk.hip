@@ -665,12 +665,9 @@ template <int NB, int RQ>
 hipError_t launch_qr_blocked_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
                                 int prepivot, int wg) {
   const size_t lds = (size_t)NB * nmax * sizeof(cplx);
-  static size_t attr = 0;
-  if (lds > 65536 && lds > attr) {
-    hipError_t e =
-        hipFuncSetAttribute((const void*)k_qr_blocked<NB, RQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  {
+    hipError_t e = ensure_dynamic_lds((const void*)k_qr_blocked<NB, RQ>, lds);
     if (e != hipSuccess) return e;
-    attr = lds;
   }
   void* args[] = {(void*)&chains, (void*)&A, (void*)&D, (void*)&prepivot};
   hipError_t e =
@@ -692,12 +689,9 @@ hipError_t launch_qr_blocked(hipStream_t s, const ChainDesc* chains, int nchains
 template <int NB, int CB>
 hipError_t launch_qr_solve_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B) {
   const size_t lds = (size_t)(CB + NB) * nmax * sizeof(cplx);
-  static size_t attr = 0;
-  if (lds > 65536 && lds > attr) {
-    hipError_t e =
-        hipFuncSetAttribute((const void*)k_qr_solve<NB, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  {
+    hipError_t e = ensure_dynamic_lds((const void*)k_qr_solve<NB, CB>, lds);
     if (e != hipSuccess) return e;
-    attr = lds;
   }
   hipLaunchKernelGGL((k_qr_solve<NB, CB>), dim3((nmax + CB - 1) / CB, nchains), dim3(256), lds, s, chains, A, B);
   return hipGetLastError();
@@ -707,12 +701,9 @@ template <int NB, int CB>
 hipError_t launch_form_Q_blocked_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A,
                                     MatRef Q) {
   const size_t lds = (size_t)(CB + NB) * nmax * sizeof(cplx);
-  static size_t attr = 0;
-  if (lds > 65536 && lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_form_Q_blocked<NB, CB>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  {
+    hipError_t e = ensure_dynamic_lds((const void*)k_form_Q_blocked<NB, CB>, lds);
     if (e != hipSuccess) return e;
-    attr = lds;
   }
   hipLaunchKernelGGL((k_form_Q_blocked<NB, CB>), dim3((nmax + CB - 1) / CB, nchains), dim3(256), lds, s, chains, A,
                      Q);

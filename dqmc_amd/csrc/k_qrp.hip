@@ -339,11 +339,9 @@ hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, i
   if (e0 != hipSuccess) return e0;
   const int cb = nmax <= 512 ? 8 : 4;
   const size_t lds = (size_t)2 * cb * nmax * sizeof(cplx);
-  static size_t attr = 0;
-  if (lds > 65536 && lds > attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_form_Q, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  {
+    hipError_t e = ensure_dynamic_lds((const void*)k_form_Q, lds);
     if (e != hipSuccess) return e;
-    attr = lds;
   }
   hipLaunchKernelGGL(k_form_Q, dim3((nmax + cb - 1) / cb, nchains), dim3(256), lds, s, chains, A, Q, cb);
   return hipGetLastError();

@@ -11,6 +11,7 @@ Slices are 1-based wherever the reference uses `current_slice` (0..M+1);
 sites are 0-based.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -104,6 +105,7 @@ class DQMC:
         self.h = h
         self.stream = uniforms if uniforms is not None else UniformStream(p.seed)
         self._log_cap = 0
+        self._pending_uniforms = False  # a sweep(sync=False) whose consumption is not yet counted
 
     def close(self):
         if getattr(self, "h", None):
@@ -111,6 +113,10 @@ class DQMC:
             self.h = None
 
     def __del__(self):
+        # at interpreter shutdown the HIP runtime may already be finalised; the process
+        # exit releases the device memory then
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -149,12 +155,21 @@ class DQMC:
         return cs.value, d.value, drift
 
     def _upload_uniforms(self, count):
+        # uniforms of an unsynchronised sweep must be consumed before the next upload
+        # (uploading resets the device cursor), else they would be handed out again
+        self._settle_uniforms()
         vals = np.ascontiguousarray(self.stream.take(count))
         check(self.lib.dqmc_set_uniforms(self.h, dptr(vals), len(vals)))
 
     def _consume_uniforms(self):
         st = self.stats()
         self.stream.advance(st.uniforms_used)
+        self._pending_uniforms = False
+
+    def _settle_uniforms(self):
+        if self._pending_uniforms:
+            check(self.lib.dqmc_synchronize(self.h))
+            self._consume_uniforms()
 
     def local_updates(self, box=None):
         """local_updates (local_updates.jl:1-39); returns the acceptance rate."""
@@ -189,12 +204,16 @@ class DQMC:
         box = self.p.box if box is None else box
         self._upload_uniforms(4 * self.p.N * self.p.slices * nsweeps)
         check(self.lib.dqmc_sweep(self.h, nsweeps, box))
+        self._pending_uniforms = True
         if sync:
             self.synchronize()
-            self._consume_uniforms()
 
     def synchronize(self):
+        """Waits for the chain's queued work; the uniforms an unsynchronised sweep drew are
+        then consumed from the host stream."""
         check(self.lib.dqmc_synchronize(self.h))
+        if self._pending_uniforms:
+            self._consume_uniforms()
 
     def set_groups(self, wg_qr=0, wg_inv=0, wg_local=0):
         check(self.lib.dqmc_ctx_set_groups(self.h, wg_qr, wg_inv, wg_local))
@@ -457,6 +476,7 @@ class Batch:
         check(self.lib.dqmc_batch_create(P, O, nc, device, ctypes.byref(h)))
         self.h = h
         self.streams = streams if streams is not None else [UniformStream(p.seed) for p in self.params]
+        self._pending_uniforms = False
 
     def close(self):
         if getattr(self, "h", None):
@@ -464,6 +484,8 @@ class Batch:
             self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -485,18 +507,24 @@ class Batch:
         check(self.lib.dqmc_batch_init(self.h))
 
     def sweep(self, nsweeps=1, sync=True):
+        self.synchronize()  # consumes the uniforms of an earlier unsynchronised sweep
         for c, p in enumerate(self.params):
             vals = np.ascontiguousarray(self.streams[c].take(4 * p.N * p.slices * nsweeps))
             check(self.lib.dqmc_batch_set_uniforms(self.h, c, dptr(vals), len(vals)))
         box = np.array([p.box for p in self.params])
         check(self.lib.dqmc_batch_sweep(self.h, nsweeps, dptr(box)))
+        self._pending_uniforms = True
         if sync:
             self.synchronize()
-            for c in range(len(self.params)):
-                self.streams[c].advance(self.stats(c).uniforms_used)
 
     def synchronize(self):
+        """Waits for every chain's queued work; the uniforms an unsynchronised sweep drew are
+        then consumed from the host streams."""
         check(self.lib.dqmc_batch_synchronize(self.h))
+        if self._pending_uniforms:
+            for c in range(len(self.params)):
+                self.streams[c].advance(self.stats(c).uniforms_used)
+            self._pending_uniforms = False
 
     def set_groups(self, wg_qr=0, wg_inv=0, wg_local=0):
         """Workgroups per chain of every shape group (0 keeps the current value)."""

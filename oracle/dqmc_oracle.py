@@ -1001,14 +1001,15 @@ def _ev_op_grad(p, op):
     return interaction_matrix_exp_op(p, op, -1.0), grads
 
 
-def action_grad_trace(m):
+def action_grad_trace(m, slices=None):
     """dS/dphi of S = S_B - log det(1 + B_M ... B_1) by the trace formula
     d log det / d phi_{a,i,l} = Re tr[(1 - G(l))_ii eV_i^-1 d_a eV_i], with G(l) the
-    Green's function at slice l (stack.jl:336) from calc_greens."""
+    Green's function at slice l (stack.jl:336) from calc_greens. `slices` (0-based) limits
+    the fermionic part to those slices (the others keep only dS_B/dphi); None = all."""
     p = m.p
     N = p.N
     g = boson_action_grad(p, m.phi)
-    for l in range(p.slices):
+    for l in (range(p.slices) if slices is None else slices):
         G = calc_greens(m, l + 1)
         for i in range(N):
             OG = np.eye(4) - G[i::N, i::N]

@@ -531,3 +531,30 @@ def test_hmc_trajectory_matches_cpu_leapfrog(amd):
     if not acc2:
         assert np.array_equal(c.hsfield, phi_before)
     c.close()
+
+
+def test_unsynchronised_sweeps_consume_the_stream(amd):
+    """sweep(sync=False) twice, synchronize, then a synchronised sweep: the host stream advances
+    by what each sweep drew, so the three sweeps see the oracle's uniforms in order."""
+    mod, DQMC = amd
+    from dqmc_amd import UniformStream
+    _say("unsynchronised sweeps L=4 M=20, 3 sweeps")
+    p = config_params(mod, 4, 20)
+    phi = np.random.default_rng(71).uniform(-1, 1, (3, p.N, p.slices))
+    c = DQMC(p, uniforms=UniformStream(171))
+    c.set_proposal_log(3 * p.N * p.slices + 16)
+    c.init(phi)
+    c.sweep(1, sync=False)
+    c.sweep(1, sync=False)
+    c.synchronize()
+    c.sweep(1)
+    o = O.DQMC(oracle_params(p), phi)
+    o.init()
+    ostream = O.UniformStream(O.host_uniforms(171, 8 * p.N * p.slices * 3 + 64))
+    olog = []
+    for _ in range(3):
+        o.sweep(ostream, olog)
+    assert _compare_logs(c.proposal_log(), olog) == 0
+    assert c.stream.consumed == ostream.pos
+    assert np.abs(c.greens - o.greens).max() < 1e-10
+    c.close()
