@@ -7,12 +7,14 @@ phi in [-1,1]^3. One step = one sweep = M update steps (propagate + local
 updates of every site), the reference's printed "sweep duration"
 (dqmc_framework.jl:259-262). Prints one JSON line.
 
-The JSON line also carries the two other legs of the metric: "batch" (aggregate sweeps/s
-of 64 independent chains of the same workload advanced together) and "grad" (dS/dphi
-evaluations per second at L=12, beta=10).
+The JSON line also carries the other legs of the metric: "batch" (aggregate sweeps/s of 64
+independent chains of the same workload advanced together, with the roofline of its dominant
+kernel), "batch_mixed" (aggregate sweeps/s of the 64-chain mixed batch: L in {6,8,10,12},
+beta in {5,10,20}, per-chain lambda and r) and "grad" (dS/dphi evaluations per second at
+L=12, beta=10).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--batch C]
-                       [--batch-sweeps S] [--grad-evals E] [--no-tune]
+                       [--batch-sweeps S] [--batch-mixed C] [--grad-evals E] [--no-tune]
 
 Both sweep legs run the autotuner (dqmc_autotune) after init, outside the timed region;
 its choice is reported under "tuning".
@@ -68,36 +70,82 @@ def measured_traffic(kernel_class):
         return None
 
 
-def cpu_baseline(p_amd, phi, slices=100):
-    """The oracle (numpy + OpenBLAS LAPACK, 1 BLAS thread) on `slices` update steps of the
-    same workload after its init; scaled to sweeps/s."""
-    from threadpoolctl import threadpool_limits
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p_amd, phi, steps=20, repeats=5):
+    """The oracle (numpy + scipy's OpenBLAS LAPACK: zgeqp3, zungqr, zgetrf, zgetri) on the same
+    workload after its init, at 1 BLAS thread (the reference's default, OMP_NUM_THREADS=1)
+    and at one thread per host core: `repeats` samples of `steps` consecutive update steps
+    (each sample holds steps / safe_mult stack recomputes, the sweep's proportion) after one
+    warm-up sample; the median sample is scaled to sweeps/s. The faster setting is `value`."""
+    import scipy.linalg  # noqa: F401  (loads the BLAS so threadpoolctl can see it)
+    from threadpoolctl import threadpool_info, threadpool_limits
     from oracle import dqmc_oracle as O
     p = O.Params(p_amd.L, p_amd.slices, delta_tau=p_amd.delta_tau, safe_mult=p_amd.safe_mult,
                  hoppings=p_amd.hoppings, mu1=p_amd.mu1, mu2=p_amd.mu2, lam=p_amd.lam, r=p_amd.r, c=p_amd.c,
                  u=p_amd.u, chkr=True, box=p_amd.box)
-    with threadpool_limits(1):
-        o = O.DQMC(p, phi)
-        o.init()
-        stream = O.UniformStream(O.host_uniforms(p_amd.seed + 7, 4 * p.N * slices + 16))
-        t0 = time.perf_counter()
-        for _ in range(slices):
-            o.update(stream)
-        dt = time.perf_counter() - t0
-    return {"value": (slices / p.slices) / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
-            "sample": "%d of the %d update steps of one sweep (incl. %d stack recomputes) after init, "
-                      "numpy/OpenBLAS oracle, 1 BLAS thread, scaled to sweeps/s" % (slices, p.slices,
-                                                                                   slices // p.safe_mult)}
+    blas = [{"api": b.get("internal_api"), "version": b.get("version"), "arch": b.get("architecture")}
+            for b in threadpool_info() if b.get("user_api") == "blas"]
+    ncores = os.cpu_count() or 1
+    res = {}
+    for threads in sorted({1, ncores}):
+        with threadpool_limits(threads):
+            o = O.DQMC(p, phi)
+            o.init()
+            stream = O.UniformStream(O.host_uniforms(p_amd.seed + 7, 4 * p.N * steps * (repeats + 1) + 16))
+            samples = []
+            for r in range(repeats + 1):
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    o.update(stream)
+                if r > 0:
+                    samples.append(time.perf_counter() - t0)
+        med = float(np.median(samples))
+        res[threads] = {"value": (steps / p.slices) / med, "cores": threads,
+                        "samples_s": [round(x, 4) for x in samples]}
+    best = max(res.values(), key=lambda r: r["value"])
+    return {"value": best["value"], "unit": "sweeps/s", "cores": best["cores"], "kind": "port",
+            "sample": "median of %d samples of %d consecutive update steps (%d stack recomputes each) of the "
+                      "L=10 beta=20 chain after init and one warm-up sample, numpy/OpenBLAS oracle, scaled to "
+                      "sweeps/s (1 sweep = %d steps)" % (repeats, steps, steps // p.safe_mult, p.slices),
+            "by_threads": {str(k): v for k, v in res.items()}, "cpu_model": _cpu_model(), "nproc": ncores,
+            "blas": blas}
 
 
-def batch_measure(amd, nchains, sweeps, warmup, tune=True):
-    """Aggregate sweeps/s of `nchains` independent chains at the same workload (L=10, beta=20),
-    advanced together by one batch (every kernel launch covers all chains; sweeps replayed
-    from HIP graphs)."""
+def mixed_params(amd, nchains=64):
+    """The mixed batch: chain c has L = [6,8,10,12][c mod 4], beta = [5,10,20][c//4 mod 3],
+    r = [-1,0,1][c//12 mod 3], lambda = 0.5 + 1.5 u_c (u_c from seed 2000 + c)."""
+    ps = []
+    for c in range(nchains):
+        L = (6, 8, 10, 12)[c % 4]
+        beta = (5, 10, 20)[(c // 4) % 3]
+        p = workload_params(amd, L=L, slices=int(round(beta / 0.1)), seed=2000 + c)
+        p.r = (-1.0, 0.0, 1.0)[(c // 12) % 3]
+        p.lam = 0.5 + 1.5 * float(np.random.default_rng(2000 + c).uniform())
+        ps.append(p)
+    return ps
+
+
+def batch_measure(amd, params, sweeps, warmup, tune=True, roofline=True):
+    """Aggregate sweeps/s of independent chains advanced together by one batch (chains of one
+    shape share every kernel launch; each shape group's sweep is replayed from HIP graphs).
+    `warmup` >= 2 untimed sweeps capture both graphs (up and down sweeps) before timing. With
+    `roofline` (one shape group), one more untimed eager sweep times the local-update kernel
+    with HIP events for the roofline of the batch's dominant kernel."""
     from dqmc_amd import Batch
-    params = [workload_params(amd, seed=2000 + c) for c in range(nchains)]
+    from dqmc_amd import _capi
+    nchains = len(params)
     b = Batch(params)
-    b.use_graph(True)  # each chain group's sweep is replayed from a captured HIP graph
+    b.use_graph(True)
     confs = [np.random.default_rng(p.seed).uniform(-1.0, 1.0, (3, p.N, p.slices)) for p in params]
     b.init(confs)
     tuning = b.autotune(reps=2) if tune else None
@@ -109,12 +157,32 @@ def batch_measure(amd, nchains, sweeps, warmup, tune=True):
     b.synchronize()
     dt = time.perf_counter() - t0
     sts = [b.stats(c) for c in range(nchains)]
+    out = {"chains": nchains, "value": nchains * sweeps / dt, "unit": "sweeps/s (all chains)",
+           "per_chain": sweeps / dt, "sweeps": sweeps, "warmup": warmup, "seconds": dt,
+           "drift_max": max(st.drift_max for st in sts),
+           "acceptance": sum(st.accepted for st in sts) / max(sum(st.proposed for st in sts), 1),
+           "shapes": sorted({(p.L, p.slices) for p in params}), "tuning": tuning}
+    if roofline and len(out["shapes"]) == 1:
+        n = params[0].n
+        acc0 = sum(st.accepted for st in sts)
+        b.lib.dqmc_batch_profile(b.h, 1 << _capi.K_LOCAL)
+        b.sweep(1)
+        ms = np.zeros(_capi.K_NCLASS)
+        cnt = np.zeros(_capi.K_NCLASS, dtype=np.int64)
+        _capi.check(b.lib.dqmc_batch_profile_read(b.h, _capi.dptr(ms), cnt.ctypes.data_as(_capi._LL),
+                                                  _capi.K_NCLASS))
+        b.lib.dqmc_batch_profile(b.h, 0)
+        acc = sum(b.stats(c).accepted for c in range(nchains)) - acc0
+        launches = int(cnt[_capi.K_LOCAL])
+        avg_ms = float(ms[_capi.K_LOCAL]) / max(launches, 1)
+        work = 32.0 * n * n * acc / max(launches, 1)
+        achieved = work / (avg_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": "local_updates (all chains per launch)", "achieved": achieved,
+                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                           "work_per_launch": work, "work_unit": "B", "avg_launch_ms": avg_ms,
+                           "launches": launches}
     b.close()
-    return {"chains": nchains, "value": nchains * sweeps / dt, "unit": "sweeps/s (all chains)",
-            "per_chain": sweeps / dt, "sweeps": sweeps, "warmup": warmup, "seconds": dt,
-            "drift_max": max(st.drift_max for st in sts),
-            "acceptance": sum(st.accepted for st in sts) / max(sum(st.proposed for st in sts), 1),
-            "tuning": tuning}
+    return out
 
 
 def grad_measure(amd, evals, warmup):
@@ -137,18 +205,44 @@ def grad_measure(amd, evals, warmup):
             "components": int(g.size), "evals": evals, "seconds": dt}
 
 
+def calibration_copy(mib):
+    """One device-to-device copy of `mib` MiB (the runtime's copy kernel): a kernel of known
+    HBM traffic that calibrates FETCH_SIZE / WRITE_SIZE counter passes. Uses the HIP runtime
+    the library links (already loaded), not a second runtime instance."""
+    import ctypes
+    from dqmc_amd import _capi
+    _capi.load()
+    hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), nbytes) or hip.hipMalloc(ctypes.byref(dst), nbytes):
+        raise RuntimeError("calibration copy: hipMalloc failed")
+    hip.hipMemset(src, 1, nbytes)
+    if hip.hipMemcpy(dst, src, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("calibration copy: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="update steps per CPU-baseline sample")
     ap.add_argument("--batch", type=int, default=64, help="chains of the batch leg (0 skips it)")
-    ap.add_argument("--batch-sweeps", type=int, default=1)
+    ap.add_argument("--batch-sweeps", type=int, default=5)
+    ap.add_argument("--batch-warmup", type=int, default=2)
+    ap.add_argument("--batch-mixed", type=int, default=64, help="chains of the mixed-batch leg (0 skips it)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (kernel profiles use this so tuning launches stay out)")
-    ap.add_argument("--grad-evals", type=int, default=3, help="0 skips the gradient leg")
+    ap.add_argument("--grad-evals", type=int, default=10, help="0 skips the gradient leg")
     ap.add_argument("--pmc-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known traffic that "
                          "calibrates FETCH_SIZE / WRITE_SIZE counter runs); 0 skips it")
@@ -163,12 +257,7 @@ def main():
     from dqmc_amd import _capi
 
     if args.pmc_copy_mib > 0:
-        import torch
-        src = torch.ones(args.pmc_copy_mib << 17, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibration_copy(args.pmc_copy_mib)
 
     p = workload_params(dqmc_amd)
     rng = np.random.default_rng(p.seed)
@@ -196,7 +285,6 @@ def main():
     chain.synchronize()
     dt = time.perf_counter() - t0
     st = chain.stats()
-    chain.stream.advance(st.uniforms_used)
     _capi.check(chain.lib.dqmc_profile_read(chain.h, _capi.dptr(ms), cnt.ctypes.data_as(_capi._LL),
                                             _capi.K_NCLASS))
     avg_ms = float(ms[dominant]) / max(int(cnt[dominant]), 1)
@@ -238,11 +326,18 @@ def main():
     }
     chain.close()
     if args.batch > 0:
-        out["batch"] = batch_measure(dqmc_amd, args.batch, args.batch_sweeps, 1, tune=not args.no_tune)
+        params = [workload_params(dqmc_amd, seed=2000 + c) for c in range(args.batch)]
+        out["batch"] = batch_measure(dqmc_amd, params, args.batch_sweeps, args.batch_warmup, tune=not args.no_tune)
+        out["batch"]["vs_single_chain"] = out["batch"]["value"] / out["value"]
+    if args.batch_mixed > 0:
+        out["batch_mixed"] = batch_measure(dqmc_amd, mixed_params(dqmc_amd, args.batch_mixed), args.batch_sweeps,
+                                           args.batch_warmup, tune=not args.no_tune, roofline=False)
     if args.grad_evals > 0:
         out["grad"] = grad_measure(dqmc_amd, args.grad_evals, 1)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p, phi, args.cpu_slices)
+        out["cpu_baseline"] = cpu_baseline(p, phi, args.cpu_steps)
+        if args.batch > 0:
+            out["batch"]["vs_cpu"] = out["batch"]["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 
 
