@@ -80,6 +80,7 @@ SIGNATURES = {
     "dqmc_gemm": [_P, _i, _i, _D, _D, _D],
     "dqmc_inverse": [_P, _D, _D, _D],
     "dqmc_boson_action": [_P, _D],
+    "dqmc_inverse_noalloc": [_i, _P, _P, _P, _P],
     "dqmc_global_update": [_P, _d, _I],
     "dqmc_effective_greens2greens": [_P, _D],
     "dqmc_tdgf_alloc": [_P],

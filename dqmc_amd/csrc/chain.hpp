@@ -215,6 +215,10 @@ hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, 
 
 // X <- X^-1 in place (Gauss-Jordan, partial pivoting); scal[S_LOGDET_A] = log|det X|.
 hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);
+// The elimination of launch_inverse alone: the inverse ends in the descriptor's M_INVT
+// matrix (X holds it with permuted columns), uses inv_bcast, the barrier words and
+// scal[S_LOGDET_A], and nothing else of the chain.
+hipError_t launch_inverse_gather(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg);
 
 // One local-update pass over all sites at `slice` (0-based). variant 1 updates G in global
 // memory after every accepted proposal, 2 keeps each workgroup's column block of G in LDS

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <tuple>
@@ -2130,3 +2131,95 @@ int dqmc_use_graph(dqmc_ctx* ctx, int enable) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// dqmc_inverse_noalloc: dqmc_inverse on caller-provided device buffers, for callers that
+// cannot allocate (code under HIP graph capture, serving loops). Its device-side scratch --
+// the kernel's descriptor, the pivot-column broadcast buffer, the group-barrier words, the
+// error word and the scalar slots -- lives in the fixed array `scratch` below, one copy per
+// device for the life of the process. Every call on a device shares that device's copy, so
+// calls that overlap on one device race; they must be ordered (one stream, or events).
+// ===========================================================================
+namespace {
+
+constexpr int kNoAllocMaxN = 1024;
+constexpr size_t kNoAllocDescBytes = (sizeof(ChainDesc) + 255) / 256 * 256;
+constexpr size_t kNoAllocScalOff = kNoAllocDescBytes;                             // kNumScalar doubles
+constexpr size_t kNoAllocBarOff = kNoAllocScalOff + 256;                          // count, gen, error
+constexpr size_t kNoAllocBcastOff = kNoAllocBarOff + 256;                         // [2][n] complex
+constexpr size_t kNoAllocBytes = kNoAllocBcastOff + (size_t)2 * kNoAllocMaxN * sizeof(cplx);
+static_assert(kNumScalar * sizeof(double) <= 256, "scalar slots exceed their scratch slot");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;  // device -> device address of `scratch`
+
+// The device address of this device's `scratch`, looked up once per device.
+int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    it = g_scratch_ptr.emplace(dev, (unsigned char*)p).first;
+  }
+  *out = it->second;
+  return 0;
+}
+
+// Builds the inverse kernel's descriptor inside the scratch: X (matrix M_TMP4) is the caller's
+// A, the gathered inverse (M_INVT) the caller's Ainv; clears the barrier and error words.
+__global__ void k_noalloc_inverse_setup(unsigned char* base, int n, cplx* A, cplx* Ainv) {
+  if (threadIdx.x != 0) return;
+  ChainDesc* d = reinterpret_cast<ChainDesc*>(base);
+  *d = ChainDesc{};
+  d->n = n;
+  d->N = n / 4;
+  d->mat[M_TMP4] = A;
+  d->mat[M_INVT] = Ainv;
+  d->scal = reinterpret_cast<double*>(base + kNoAllocScalOff);
+  unsigned int* bar = reinterpret_cast<unsigned int*>(base + kNoAllocBarOff);
+  bar[0] = 0u;
+  bar[1] = 0u;
+  d->bar_count = bar;
+  d->bar_gen = bar + 1;
+  d->error = reinterpret_cast<int*>(bar + 2);
+  *d->error = 0;
+  d->inv_bcast = reinterpret_cast<cplx*>(base + kNoAllocBcastOff);
+  for (int k = 0; k < kNumScalar; ++k) d->scal[k] = 0.0;
+}
+
+// log|det A| to the caller (NaN after a device-side failure, a barrier timeout).
+__global__ void k_noalloc_inverse_finish(const unsigned char* base, double* logabsdet) {
+  if (threadIdx.x != 0) return;
+  const ChainDesc* d = reinterpret_cast<const ChainDesc*>(base);
+  *logabsdet = (*d->error != 0) ? __builtin_nan("") : d->scal[S_LOGDET_A];
+}
+
+}  // namespace
+
+extern "C" int dqmc_inverse_noalloc(int n, double* A, double* Ainv, double* logabsdet, void* stream) {
+  if (n < 1 || n > kNoAllocMaxN) return fail(DQMC_E_ARG, "dqmc_inverse_noalloc: n must be in 1..1024");
+  if (!A || !Ainv || A == Ainv) return fail(DQMC_E_ARG, "dqmc_inverse_noalloc: A and Ainv must be distinct buffers");
+  unsigned char* base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int wg = std::max((n + 511) / 512, std::min(std::max(1, n / 8), kMaxWG));
+  hipLaunchKernelGGL(k_noalloc_inverse_setup, dim3(1), dim3(64), 0, s, base, n, (cplx*)A, (cplx*)Ainv);
+  HIPCHK(hipGetLastError());
+  const ChainDesc* desc = reinterpret_cast<const ChainDesc*>(base);
+  LCHK(launch_inverse_gather(s, desc, 1, n, NM(M_TMP4), wg));
+  if (logabsdet) {
+    hipLaunchKernelGGL(k_noalloc_inverse_finish, dim3(1), dim3(64), 0, s, (const unsigned char*)base, logabsdet);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}

@@ -475,15 +475,18 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const ChainDesc* chains, Mat
   }
 }
 
-hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg) {
-  if (nmax > kMaxN) return hipErrorInvalidValue;
+hipError_t launch_inverse_gather(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg) {
+  if (nmax > kMaxN || wg < 1) return hipErrorInvalidValue;
   if ((nmax + wg - 1) / wg > kMaxN / 2) return hipErrorInvalidValue;
   void* args[] = {(void*)&chains, (void*)&X};
   hipError_t e = coop_launch((const void*)k_gj_inverse, dim3(wg, nchains), args, s);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return e;
-  }
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
+}
+
+hipError_t launch_inverse(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef X, int wg) {
+  hipError_t e = launch_inverse_gather(s, chains, nchains, nmax, X, wg);
+  if (e != hipSuccess) return e;
   return launch_copy(s, chains, nchains, nmax, X, MatRef{MK_NAMED, M_INVT});
 }
 

@@ -188,6 +188,17 @@ int dqmc_inverse(dqmc_ctx* ctx, const double* A, double* Ainv, double* logabsdet
 /* calc_boson_action (action.jl:1-52) of the current field */
 int dqmc_boson_action(dqmc_ctx* ctx, double* S);
 
+/* No reference counterpart: dqmc_inverse for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Needs no context and allocates nothing: A, Ainv (n x n complex, column-major, interleaved)
+ * and logabsdet (one double, may be NULL) are device pointers the caller owns; the
+ * library's scratch (about 33 KB) is a fixed per-device array. Limit: 1 <= n <= 1024.
+ * A is overwritten (it holds the inverse with permuted columns afterwards); Ainv = inv(A),
+ * *logabsdet = log|det A| (NaN after a device-side failure). Asynchronous on `stream` (a
+ * hipStream_t; NULL = the null stream). Calls that overlap on one device share the scratch
+ * and race: order them on one stream. Same kernel and results as dqmc_inverse. */
+int dqmc_inverse_noalloc(int n, double* A, double* Ainv, double* logabsdet, void* stream);
+
 /* global_update (global_updates.jl:19-58), callable when current_slice == M and
  * direction == -1 (update, dqmc_framework.jl:507-512): shifts every component of
  * phi by -b + 2 b u with three uniforms from the stream (b = box_global), rebuilds
