@@ -89,6 +89,15 @@ SIGNATURES = {
     "dqmc_get_tdgf": [_P, _i, _D, _D],
     "dqmc_get_tdgf_stack": [_P, _i, _i, _D, _D, _D],
     "dqmc_tdgf_direct": [_P, _i, _D],
+    "dqmc_measure_greens": [_P, _D],
+    "dqmc_measure_current_greens": [_P, _D],
+    "dqmc_set_etgf": [_P, _D],
+    "dqmc_set_hopping_matrix": [_P, _D],
+    "dqmc_measure_etpc": [_P, _D, _D],
+    "dqmc_measure_etcdc": [_P, _D, _D],
+    "dqmc_measure_zfpc": [_P, _D, _D],
+    "dqmc_measure_zfcdc": [_P, _D, _D],
+    "dqmc_measure_zfccc": [_P, _D, _D, _D],
     "dqmc_grad": [_P, _D, _D],
     "dqmc_hmc_trajectory": [_P, _D, _i, _d, _d, _I, _D],
     "dqmc_batch_create": [ctypes.POINTER(Params), ctypes.POINTER(Operators), _i, _i, ctypes.POINTER(_P)],

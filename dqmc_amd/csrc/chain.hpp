@@ -265,4 +265,15 @@ hipError_t launch_global_restore(hipStream_t s, const double* flag, const Restor
 hipError_t launch_grad_slice(hipStream_t s, const ChainDesc* chains, int nchains, int Nmax, int slice,
                              double* const* grad);
 
+// Fermionic observables of one chain (fermion_measurements.jl:388-899): kind 0 pairing
+// (etpc / zfpc), 1 charge density (etcdc / zfcdc), 2 current-current (zfccc). Sums over
+// ntau imaginary times of Gt0 + tau sGt0 and G0t + tau sG0t (sGt0 = 0: the equal-time form on
+// one G), with Gtau and G0 the equal-time Green's function, over origins and displacements;
+// res[2][N] complex = the two L x L result arrays (index y + L x). tnb: [2 dirs][4 flavours]
+// [N][2] hopping entries (kind 2). part: fermion_obs_blocks(ntau, N) x 2N complex scratch.
+int fermion_obs_blocks(int ntau, int N);
+hipError_t launch_fermion_obs(hipStream_t s, int kind, int L, int ntau, const cplx* Gtau, const cplx* G0,
+                              const cplx* Gt0, size_t sGt0, const cplx* G0t, size_t sG0t, const cplx* tnb, cplx* part,
+                              cplx* res);
+
 }  // namespace dqmc

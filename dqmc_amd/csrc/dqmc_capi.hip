@@ -67,6 +67,14 @@ struct ChainMem {
   double* tdgf_vec = nullptr;
   // global-update backup (gb_u/d/t_stack, gb_greens, gb_hsfield; stack.jl:134-143), one block
   double* gb = nullptr;
+  // fermionic measurements (allocated on first use): the equal-time Green's function the
+  // observables read (etgf), the hopping entries of the current operator, partial bins, results
+  cplx* meas = nullptr;
+  cplx* meas_etgf = nullptr;
+  cplx* meas_tnb = nullptr;
+  cplx* meas_part = nullptr;
+  cplx* meas_res = nullptr;
+  bool has_etgf = false, has_hopping = false;
 };
 
 struct Group {
@@ -303,8 +311,10 @@ void chain_free(ChainMem& cm) {
   if (cm.uniforms) (void)hipFree(cm.uniforms);
   if (cm.log_buf) (void)hipFree(cm.log_buf);
   for (void* p : {(void*)cm.grad, (void*)cm.grad_ptr, (void*)cm.mom, (void*)cm.phi_bak, (void*)cm.tdgf_mat,
-                  (void*)cm.tdgf_vec, (void*)cm.gb})
+                  (void*)cm.tdgf_vec, (void*)cm.gb, (void*)cm.meas})
     if (p) (void)hipFree(p);
+  cm.meas = cm.meas_etgf = cm.meas_tnb = cm.meas_part = cm.meas_res = nullptr;
+  cm.has_etgf = cm.has_hopping = false;
   cm.tdgf_mat = nullptr;
   cm.tdgf_vec = nullptr;
   cm.gb = nullptr;
@@ -1777,6 +1787,193 @@ int dqmc_hmc_trajectory(dqmc_ctx* ctx, const double* momenta, int nsteps, double
   RC(batch_sync(b));
   if (accepted) *accepted = acc;
   if (dH) *dH = dh;
+  return 0;
+}
+
+// ---- fermionic measurements (fermion_measurements.jl:216-229, 363-957) --------------------
+}  // extern "C"
+namespace {
+
+int meas_alloc(ChainMem& cm) {
+  if (cm.meas) return 0;
+  const size_t nn = (size_t)cm.n * cm.n, N = (size_t)cm.N;
+  const size_t blocks = (size_t)std::max(fermion_obs_blocks(cm.M, cm.N), fermion_obs_blocks(1, cm.N));
+  const size_t tot = nn + 16 * N + blocks * 2 * N + 2 * N;
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, tot * sizeof(cplx)));
+  cm.meas = (cplx*)p;
+  cm.meas_etgf = cm.meas;
+  cm.meas_tnb = cm.meas_etgf + nn;
+  cm.meas_part = cm.meas_tnb + 16 * N;
+  cm.meas_res = cm.meas_part + blocks * 2 * N;
+  return 0;
+}
+
+// one observable family on the device, the two L x L result arrays to the host (complex, or
+// only the real parts when `real_only`)
+int meas_run(dqmc_ctx* ctx, int kind, bool zf, double* out0, double* out1, bool real_only) {
+  dqmc_batch* b = ctx->b;
+  ChainMem& cm = b->chains[0];
+  Group& g = b->groups[0];
+  if (!cm.has_etgf && !(kind == 0 && zf))
+    return fail(DQMC_E_STATE, "no equal-time Green's function: call dqmc_measure_greens or dqmc_set_etgf first");
+  if (zf && !cm.tdgf_mat) return fail(DQMC_E_STATE, "zero-frequency observables need dqmc_tdgf_alloc + dqmc_measure_tdgfs");
+  if (kind == 2 && !cm.has_hopping) return fail(DQMC_E_STATE, "current correlations need dqmc_set_hopping_matrix");
+  int rc = meas_alloc(cm);
+  if (rc) return rc;
+  const size_t nn = (size_t)cm.n * cm.n;
+  const cplx* E = cm.meas_etgf;
+  if (!zf) {
+    LCHK(launch_fermion_obs(g.stream, kind, cm.L, 1, E, E, E, 0, E, 0, cm.meas_tnb, cm.meas_part, cm.meas_res));
+  } else {
+    Ops o(g);
+    const cplx* gt0 = cm.tdgf_mat + nn * (size_t)o.GT0(1).idx;
+    const cplx* g0t = cm.tdgf_mat + nn * (size_t)o.G0T(1).idx;
+    LCHK(launch_fermion_obs(g.stream, kind, cm.L, cm.M, E, E, gt0, nn, g0t, nn, cm.meas_tnb, cm.meas_part,
+                            cm.meas_res));
+  }
+  std::vector<cplx> res((size_t)2 * cm.N);
+  HIPCHK(hipMemcpyAsync(res.data(), cm.meas_res, res.size() * sizeof(cplx), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  RC(check_device_error(b));
+  double* outs[2] = {out0, out1};
+  for (int k = 0; k < 2; ++k) {
+    if (!outs[k]) continue;
+    for (int r = 0; r < cm.N; ++r) {
+      const cplx v = res[(size_t)k * cm.N + r];
+      if (real_only) {
+        outs[k][r] = v.re;
+      } else {
+        outs[k][2 * r] = v.re;
+        outs[k][2 * r + 1] = v.im;
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+extern "C" {
+
+int dqmc_set_hopping_matrix(dqmc_ctx* ctx, const double* T) {
+  CTX_CHECK(ctx);
+  if (!T) return fail(DQMC_E_ARG, "null hopping matrix");
+  ChainMem& cm = ctx->b->chains[0];
+  RC(batch_sync(ctx->b));
+  RC(meas_alloc(cm));
+  const int L = cm.L, N = cm.N, n = cm.n;
+  std::vector<double> h((size_t)2 * 16 * N);
+  for (int d = 0; d < 2; ++d)
+    for (int f = 0; f < 4; ++f)
+      for (int i = 0; i < N; ++i) {
+        const int x = i / L, y = i - x * L;
+        const int j = d == 0 ? y + L * ((x + 1) % L) : (y + 1) % L + L * x;
+        const size_t a = (size_t)(f * N + j) * n + (f * N + i);  // T[fN + i, fN + j]
+        const size_t c = (size_t)(f * N + i) * n + (f * N + j);  // T[fN + j, fN + i]
+        double* o = h.data() + 4 * ((size_t)(d * 4 + f) * N + i);
+        o[0] = T[2 * a];
+        o[1] = T[2 * a + 1];
+        o[2] = T[2 * c];
+        o[3] = T[2 * c + 1];
+      }
+  HIPCHK(hipMemcpy(cm.meas_tnb, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  cm.has_hopping = true;
+  return 0;
+}
+
+int dqmc_measure_greens(dqmc_ctx* ctx, double* G) {
+  // measure_greens (fermion_measurements.jl:225-229): calc_greens(mc, 1) (from scratch, as
+  // calc_greens_and_logdet :1061-1101), then effective_greens2greens! (:1125-1141)
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  Group& g = b->groups[0];
+  ChainMem& cm = b->chains[0];
+  RC(meas_alloc(cm));
+  Ops o(g);
+  RC(o.bchain(1, g.M, B_DAG_LEFT, NM(M_UR), NV(V_DR), NM(M_TR)));
+  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_UL), NOVEC));
+  LCHK(launch_identity(g.stream, g.d_desc, 1, g.n, NM(M_TL), NOVEC));
+  LCHK(launch_ones_vec(g.stream, g.d_desc, 1, g.n, NV(V_DL)));
+  double* sc = cm.d.scal;
+  HIPCHK(hipMemcpyAsync(sc + S_LOGDET_SAVE, sc + S_LOGDET, sizeof(double), hipMemcpyDeviceToDevice, g.stream));
+  RC(o.copy(NM(M_GW), NM(M_G)));
+  RC(o.calculate_greens(true));
+  RC(o.effective_greens(NM(M_G)));
+  HIPCHK(hipMemcpyAsync(cm.meas_etgf, cm.d.mat[M_G], sizeof(cplx) * cm.n * cm.n, hipMemcpyDeviceToDevice, g.stream));
+  RC(o.copy(NM(M_G), NM(M_GW)));
+  HIPCHK(hipMemcpyAsync(sc + S_LOGDET, sc + S_LOGDET_SAVE, sizeof(double), hipMemcpyDeviceToDevice, g.stream));
+  cm.has_etgf = true;
+  if (G) HIPCHK(hipMemcpyAsync(G, cm.meas_etgf, sizeof(cplx) * cm.n * cm.n, hipMemcpyDeviceToHost, g.stream));
+  return batch_sync(b);
+}
+
+int dqmc_measure_current_greens(dqmc_ctx* ctx, double* G) {
+  // _measure_observables! (dqmc_framework.jl:378-382): wrap_greens(mc, mc.s.greens,
+  // current_slice, 1), then effective_greens2greens!; the chain's G is left unchanged
+  CTX_CHECK(ctx);
+  dqmc_batch* b = ctx->b;
+  Group& g = b->groups[0];
+  ChainMem& cm = b->chains[0];
+  if (g.cur < 1 || g.cur > g.M) return fail(DQMC_E_STATE, "the chain is not at a slice 1..M");
+  RC(meas_alloc(cm));
+  Ops o(g);
+  RC(o.copy(NM(M_GRAD), NM(M_G)));
+  RC(o.wrap(NM(M_GRAD), g.cur, 1));
+  RC(o.effective_greens(NM(M_GRAD)));
+  HIPCHK(hipMemcpyAsync(cm.meas_etgf, cm.d.mat[M_GRAD], sizeof(cplx) * cm.n * cm.n, hipMemcpyDeviceToDevice,
+                        g.stream));
+  cm.has_etgf = true;
+  if (G) HIPCHK(hipMemcpyAsync(G, cm.meas_etgf, sizeof(cplx) * cm.n * cm.n, hipMemcpyDeviceToHost, g.stream));
+  return batch_sync(b);
+}
+
+int dqmc_set_etgf(dqmc_ctx* ctx, const double* G) {
+  CTX_CHECK(ctx);
+  if (!G) return fail(DQMC_E_ARG, "null G");
+  ChainMem& cm = ctx->b->chains[0];
+  RC(batch_sync(ctx->b));
+  RC(meas_alloc(cm));
+  HIPCHK(hipMemcpy(cm.meas_etgf, G, sizeof(cplx) * cm.n * cm.n, hipMemcpyHostToDevice));
+  cm.has_etgf = true;
+  return 0;
+}
+
+int dqmc_measure_etpc(dqmc_ctx* ctx, double* plus, double* minus) {
+  CTX_CHECK(ctx);
+  return meas_run(ctx, 0, false, minus, plus, true);
+}
+int dqmc_measure_zfpc(dqmc_ctx* ctx, double* plus, double* minus) {
+  CTX_CHECK(ctx);
+  return meas_run(ctx, 0, true, minus, plus, true);
+}
+int dqmc_measure_etcdc(dqmc_ctx* ctx, double* plus, double* minus) {
+  CTX_CHECK(ctx);
+  return meas_run(ctx, 1, false, minus, plus, false);
+}
+int dqmc_measure_zfcdc(dqmc_ctx* ctx, double* plus, double* minus) {
+  CTX_CHECK(ctx);
+  return meas_run(ctx, 1, true, minus, plus, false);
+}
+int dqmc_measure_zfccc(dqmc_ctx* ctx, double* xx, double* yy, double* sfdensity) {
+  CTX_CHECK(ctx);
+  ChainMem& cm = ctx->b->chains[0];
+  const int L = cm.L;
+  std::vector<double> bx((size_t)2 * cm.N), by((size_t)2 * cm.N);
+  RC(meas_run(ctx, 2, true, bx.data(), by.data(), false));
+  if (xx) memcpy(xx, bx.data(), bx.size() * sizeof(double));
+  if (yy) memcpy(yy, by.data(), by.size() * sizeof(double));
+  if (sfdensity) {
+    // measure_sfdensity (:911-923): rfft of real(zfccc_xx); 1/4 (Re q=(ky 0, kx 1) - Re q=(ky 1, kx 0))
+    const double w = 2.0 * M_PI / L;
+    double qx = 0.0, qy = 0.0;
+    for (int x = 0; x < L; ++x)
+      for (int y = 0; y < L; ++y) {
+        const double v = bx[2 * ((size_t)y + (size_t)L * x)];
+        qx += v * std::cos(w * x);
+        qy += v * std::cos(w * y);
+      }
+    *sfdensity = 0.25 * (qx - qy);
+  }
   return 0;
 }
 

@@ -454,6 +454,68 @@ class DQMC:
         check(self.lib.dqmc_tdgf_direct(self.h, slice_, _cmat_ptr(G)))
         return G
 
+    # -- fermionic observables (fermion_measurements.jl:216-229, 363-957) ------------------
+    def measure_greens(self):
+        """measure_greens (fermion_measurements.jl:225-229): G(1) from scratch, effective -> true;
+        also kept on the device as the equal-time Green's function of the observables."""
+        G = np.zeros((self.p.n, self.p.n), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_measure_greens(self.h, _cmat_ptr(G)))
+        return G
+
+    def measure_current_greens(self):
+        """The greens observable of _measure_observables! (dqmc_framework.jl:378-382): the
+        chain's G wrapped up from current_slice, effective -> true (kept on the device too)."""
+        G = np.zeros((self.p.n, self.p.n), dtype=np.complex128, order="F")
+        check(self.lib.dqmc_measure_current_greens(self.h, _cmat_ptr(G)))
+        return G
+
+    def set_etgf(self, G):
+        """Use a host equal-time Green's function for the observables."""
+        G = _as_fcomplex(G, self.p.n)
+        check(self.lib.dqmc_set_etgf(self.h, _cmat_ptr(G)))
+
+    def _obs_pair(self, fn, real):
+        L = self.p.L
+        if real:
+            a, b = np.zeros((L, L), order="F"), np.zeros((L, L), order="F")
+            check(fn(self.h, dptr(a), dptr(b)))
+        else:
+            a = np.zeros((L, L), dtype=np.complex128, order="F")
+            b = np.zeros((L, L), dtype=np.complex128, order="F")
+            check(fn(self.h, _cmat_ptr(a), _cmat_ptr(b)))
+        return a, b
+
+    def measure_etpc(self):
+        """measure_etpc! (fermion_measurements.jl:388-470): (etpc_plus, etpc_minus), L x L [y, x]."""
+        return self._obs_pair(self.lib.dqmc_measure_etpc, True)
+
+    def measure_zfpc(self):
+        """measure_zfpc! (fermion_measurements.jl:500-562) on the last measure_tdgfs."""
+        return self._obs_pair(self.lib.dqmc_measure_zfpc, True)
+
+    def measure_etcdc(self):
+        """measure_etcdc! (fermion_measurements.jl:592-657): (etcdc_plus, etcdc_minus)."""
+        return self._obs_pair(self.lib.dqmc_measure_etcdc, False)
+
+    def measure_zfcdc(self):
+        """measure_zfcdc! (fermion_measurements.jl:688-778) with greens = the etgf."""
+        return self._obs_pair(self.lib.dqmc_measure_zfcdc, False)
+
+    def measure_zfccc(self):
+        """measure_zfccc! (fermion_measurements.jl:814-899) with greens = the etgf, and
+        measure_sfdensity (:911-923): (zfccc_xx, zfccc_yy, sfdensity)."""
+        if not getattr(self, "_hopping_set", False):
+            from .model import hopping_matrix
+            T = hopping_matrix(self.p)
+            check(self.lib.dqmc_set_hopping_matrix(self.h, _cmat_ptr(T)))
+            self._hopping_set = True
+        L = self.p.L
+        xx = np.zeros((L, L), dtype=np.complex128, order="F")
+        yy = np.zeros((L, L), dtype=np.complex128, order="F")
+        rho = ctypes.c_double()
+        check(self.lib.dqmc_measure_zfccc(self.h, _cmat_ptr(xx), _cmat_ptr(yy), ctypes.byref(rho)))
+        return xx, yy, rho.value
+
     def grad(self):
         """dS/dphi (3, N, M) and S = S_B + log|det G| of the current field."""
         g = np.zeros((3, self.p.N, self.p.slices), order="F")

@@ -199,6 +199,16 @@ def _hopping_blocks(p):
     return blocks
 
 
+def hopping_matrix(p):
+    """mc.l.hopping_matrix (init_hopping_matrix, hoppings.jl:16-95 / :158-254): the n x n
+    block-diagonal hopping matrix over the flavour blocks (x, y, x, y), complex128."""
+    N = p.N
+    T = np.zeros((p.n, p.n), dtype=np.complex128)
+    for b, Tb in enumerate(_hopping_blocks(p)):
+        T[b * N:(b + 1) * N, b * N:(b + 1) * N] = Tb
+    return np.asfortranarray(T)
+
+
 def _plaquette_blocks(p, prefac, sign):
     """4x4 blocks of exp(-sign * prefac * dtau * T_plaquette) for every (group, block, plaquette).
 

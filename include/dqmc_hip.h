@@ -229,6 +229,36 @@ int dqmc_get_tdgf_stack(dqmc_ctx* ctx, int which, int k, double* U, double* D, d
 /* calc_tdgf_direct (fermion_measurements.jl:1553-1576): G(slice, 1) from scratch */
 int dqmc_tdgf_direct(dqmc_ctx* ctx, int slice, double* G);
 
+/* ---- fermionic observables ------------------------------------------------ *
+ * The observables read an equal-time Green's function kept on the device (the reference's
+ * etgf, measure_framework.jl:716-720) and, for the zero-frequency forms, the time-displaced
+ * Green's functions of the last dqmc_measure_tdgfs. Result arrays are the reference's
+ * L x L arrays [y+1, x+1] in column-major order (displacement (x, y) at index y + L x):
+ * real for the pairing correlations, complex (interleaved) for the others. */
+/* measure_greens (fermion_measurements.jl:225-229): calc_greens(mc, 1) from scratch and
+ * effective_greens2greens!; kept as the etgf; G (host copy, n x n) optional */
+int dqmc_measure_greens(dqmc_ctx* ctx, double* G);
+/* the greens observable of _measure_observables! (dqmc_framework.jl:378-382): wrap_greens of
+ * the chain's G at current_slice in direction +1, then effective_greens2greens!; kept as the
+ * etgf (the chain's G is unchanged); G optional */
+int dqmc_measure_current_greens(dqmc_ctx* ctx, double* G);
+/* a host etgf (measure_framework.jl:716-717: greens loaded from disk) */
+int dqmc_set_etgf(dqmc_ctx* ctx, const double* G);
+/* mc.l.hopping_matrix (init_hopping_matrix, hoppings.jl:16-95 / :158-254), n x n complex:
+ * the hopping amplitudes of the current operator of dqmc_measure_zfccc */
+int dqmc_set_hopping_matrix(dqmc_ctx* ctx, const double* T);
+/* measure_etpc! (fermion_measurements.jl:388-470) on the etgf: etpc_plus / etpc_minus */
+int dqmc_measure_etpc(dqmc_ctx* ctx, double* plus, double* minus);
+/* measure_etcdc! (fermion_measurements.jl:592-657) on the etgf (complex) */
+int dqmc_measure_etcdc(dqmc_ctx* ctx, double* plus, double* minus);
+/* measure_zfpc! (fermion_measurements.jl:500-562) on the device Gt0 */
+int dqmc_measure_zfpc(dqmc_ctx* ctx, double* plus, double* minus);
+/* measure_zfcdc! (fermion_measurements.jl:688-778) with greens = the etgf (complex) */
+int dqmc_measure_zfcdc(dqmc_ctx* ctx, double* plus, double* minus);
+/* measure_zfccc! (fermion_measurements.jl:814-899) with greens = the etgf (complex xx, yy),
+ * and measure_sfdensity (:911-923) of the result (optional) */
+int dqmc_measure_zfccc(dqmc_ctx* ctx, double* xx, double* yy, double* sfdensity);
+
 /* No reference counterpart: dS/dphi for every (component, site, slice) of
  * S(phi) = S_B(phi) - log det(1 + B_M...B_1), computed from the Green's
  * functions of one stabilised pass over all slices. Also returns S (optional). */

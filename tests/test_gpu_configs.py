@@ -249,6 +249,16 @@ def test_config4_sweep_tdgf_hmc(amd):
     for tau in (2, 100, 200, 201, 300, 400):
         a, b = c.tdgf(tau)
         assert np.isfinite(a).all() and np.isfinite(b).all() and np.abs(a).max() < 10 and np.abs(b).max() < 10, tau
+    # the observables built on them (fermion_measurements.jl:500-923), all 400 slices
+    t0 = time.perf_counter()
+    c.measure_greens()
+    zp, zm = c.measure_zfpc()
+    cp, cm = c.measure_zfcdc()
+    xx, yy, rho = c.measure_zfccc()
+    _say("  zfpc + zfcdc + zfccc %.2f s, sfdensity %.6g" % (time.perf_counter() - t0, rho))
+    for arr in (zp, zm, cp, cm, xx, yy):
+        assert arr.shape == (16, 16) and np.isfinite(arr).all()
+    assert math.isfinite(rho)
     c.deallocate_tdgfs()
     # one HMC trajectory (3 leapfrog steps of 0.02) from the swept field
     _say("L=16 M=400: one HMC trajectory, 3 leapfrog steps")
