@@ -24,7 +24,9 @@ class Params(ctypes.Structure):
 class Operators(ctypes.Structure):
     _fields_ = [("plaq_sites", ctypes.POINTER(ctypes.c_int)), ("plaq", ctypes.POINTER(ctypes.c_double)),
                 ("expmu", ctypes.POINTER(ctypes.c_double)), ("hop_exp", ctypes.POINTER(ctypes.c_double)),
-                ("hop_exp_inv", ctypes.POINTER(ctypes.c_double)), ("neighbors", ctypes.POINTER(ctypes.c_int))]
+                ("hop_exp_inv", ctypes.POINTER(ctypes.c_double)), ("neighbors", ctypes.POINTER(ctypes.c_int)),
+                ("n_groups", ctypes.c_int), ("group_offsets", ctypes.POINTER(ctypes.c_int)),
+                ("bond_sites", ctypes.POINTER(ctypes.c_int)), ("bond_blocks", ctypes.POINTER(ctypes.c_double))]
 
 
 class Stats(ctypes.Structure):

@@ -9,6 +9,9 @@
 
 namespace dqmc {
 
+// largest number of bond groups of the generic checkerboard
+constexpr int kMaxBondGroups = 16;
+
 // Named n x n complex work matrices of a chain.
 enum MatId : int {
   M_G = 0,     // equal-time Green's function (mc.s.greens)
@@ -48,7 +51,7 @@ struct VecRef {
 
 struct ChainDesc {
   // model
-  int L, N, n, M, safe_mult, nelem, chkr;  // chkr: 1 = Assaad checkerboard, 0 = dense hopping
+  int L, N, n, M, safe_mult, nelem, chkr;  // chkr: 1 = Assaad checkerboard, 2 = generic, 0 = dense hopping
   int edrun;
   int plaq_uniform;  // 1: every plaquette of a (variant, group, block) has the same 4x4 block
   double dtau, lam, r, c, u;
@@ -58,6 +61,11 @@ struct ChainDesc {
   const double* expmu;    // [n] exp(+dtau mu)   (chkr_mu diagonal)
   const double* expmu_inv;
   const int* nb;          // [4][N] neighbours (up, right, down, left)
+  // generic checkerboard: bonds of group g are grp_off[g] .. grp_off[g+1]-1
+  int ngroups, nbonds;
+  const int* grp_off;     // [ngroups + 1]
+  const int* bond_sites;  // [nbonds][2] (src, trg)
+  const cplx* bond_blocks;  // [4 variants][nbonds][4 blocks][2][2] row-major 2x2
   // state
   double* phi;            // (3, N, M) column-major
   cplx* mat[kNumMat];
@@ -155,9 +163,10 @@ enum BOp : int { B_LEFT = 0, B_RIGHT, B_INV_LEFT, B_INV_RIGHT, B_DAG_LEFT,
                  B_EFF_RIGHT, B_EFF_INV_LEFT };
 
 // X <- op(B_slice) X  or  X op(B_slice) with the checkerboard operators; slice is 0-based
-// and the same for all chains, or taken per chain from `slices` when non-null.
+// and the same for all chains, or taken per chain from `slices` when non-null. ngroups = 0:
+// the Assaad checkerboard (plaquettes); > 0: the generic one with that many bond groups.
 hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
-                               BOp op, int slice, const int* slices);
+                               BOp op, int slice, const int* slices, int ngroups = 0);
 // only the eV factor: mode 0 N, 1 T, 2 C; rows = apply to rows (from the right)
 hipError_t launch_apply_eV(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
                            bool rows, int mode, double power, int slice, const int* slices);

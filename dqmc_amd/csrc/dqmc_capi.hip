@@ -79,6 +79,7 @@ struct ChainMem {
 
 struct Group {
   int n, N, L, M, s, nelem, chkr;
+  int ngroups = 0;  // bond groups of the generic checkerboard (0: Assaad or dense)
   bool all_checks;
   std::vector<int> chains;  // indices into Batch::chains
   ChainDesc* d_desc = nullptr;
@@ -153,10 +154,32 @@ int validate(const dqmc_params* p, const dqmc_operators* o) {
   if (p->L < 2 || p->slices < 1 || p->safe_mult < 1) return fail(DQMC_E_ARG, "bad L/slices/safe_mult");
   if (p->slices % p->safe_mult != 0) return fail(DQMC_E_ARG, "slices must be a multiple of safe_mult");
   if (4 * p->L * p->L > 1024) return fail(DQMC_E_ARG, "n = 4L^2 must be <= 1024");
-  if (p->checkerboard && (p->L % 2 != 0)) return fail(DQMC_E_ARG, "checkerboard needs even L");
+  if (p->checkerboard < 0 || p->checkerboard > 2) return fail(DQMC_E_ARG, "checkerboard must be 0, 1 or 2");
+  if (p->checkerboard == 1 && (p->L % 2 != 0)) return fail(DQMC_E_ARG, "the Assaad checkerboard needs even L");
   if (!o->neighbors) return fail(DQMC_E_ARG, "neighbors required");
-  if (p->checkerboard && (!o->plaq_sites || !o->plaq || !o->expmu))
+  if (p->checkerboard == 1 && (!o->plaq_sites || !o->plaq || !o->expmu))
     return fail(DQMC_E_ARG, "checkerboard operators required");
+  if (p->checkerboard == 2) {
+    if (!o->group_offsets || !o->bond_sites || !o->bond_blocks || !o->expmu)
+      return fail(DQMC_E_ARG, "generic checkerboard operators required");
+    if (o->n_groups < 1 || o->n_groups > kMaxBondGroups)
+      return fail(DQMC_E_ARG, "generic checkerboard: 1 to 16 bond groups");
+    const int N = p->L * p->L;
+    if (o->group_offsets[0] != 0) return fail(DQMC_E_ARG, "generic checkerboard: group_offsets[0] must be 0");
+    for (int g = 0; g < o->n_groups; ++g)
+      if (o->group_offsets[g + 1] < o->group_offsets[g]) return fail(DQMC_E_ARG, "group_offsets must not decrease");
+    const int nb = o->group_offsets[o->n_groups];
+    for (int g = 0; g < o->n_groups; ++g) {
+      std::vector<char> used(N, 0);
+      for (int q = o->group_offsets[g]; q < o->group_offsets[g + 1]; ++q)
+        for (int k = 0; k < 2; ++k) {
+          const int st = o->bond_sites[2 * q + k];
+          if (st < 0 || st >= N) return fail(DQMC_E_ARG, "generic checkerboard: bond site out of range");
+          if (used[st]++) return fail(DQMC_E_ARG, "generic checkerboard: a group touches a site twice");
+        }
+    }
+    if (nb < 1) return fail(DQMC_E_ARG, "generic checkerboard: no bonds");
+  }
   if (!p->checkerboard && (!o->hop_exp || !o->hop_exp_inv)) return fail(DQMC_E_ARG, "dense operators required");
   return 0;
 }
@@ -263,7 +286,32 @@ int chain_create(ChainMem& cm, const dqmc_params* p, const dqmc_operators* o) {
   ALLOC(nbd, int, 4 * N);
   HIPCHK(hipMemcpy(nbd, o->neighbors, sizeof(int) * 4 * N, hipMemcpyHostToDevice));
   d.nb = nbd;
-  if (p->checkerboard) {
+  if (p->checkerboard == 2) {
+    const int G = o->n_groups, nbonds = o->group_offsets[G];
+    int *go, *bs;
+    cplx* bb;
+    double *em, *emi;
+    ALLOC(go, int, G + 1);
+    ALLOC(bs, int, 2 * nbonds);
+    ALLOC(bb, cplx, (size_t)4 * nbonds * 16);
+    ALLOC(em, double, n);
+    ALLOC(emi, double, n);
+    HIPCHK(hipMemcpy(go, o->group_offsets, sizeof(int) * (G + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(bs, o->bond_sites, sizeof(int) * 2 * nbonds, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(bb, o->bond_blocks, sizeof(cplx) * 4 * nbonds * 16, hipMemcpyHostToDevice));
+    std::vector<double> inv(n);
+    for (int i = 0; i < n; ++i) inv[i] = 1.0 / o->expmu[i];
+    HIPCHK(hipMemcpy(em, o->expmu, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(emi, inv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    d.ngroups = G;
+    d.nbonds = nbonds;
+    d.grp_off = go;
+    d.bond_sites = bs;
+    d.bond_blocks = bb;
+    d.expmu = em;
+    d.expmu_inv = emi;
+    d.plaq_uniform = 0;
+  } else if (p->checkerboard) {
     int* ps;
     cplx* pl;
     double *em, *emi;
@@ -432,7 +480,7 @@ struct Ops {
   int applyB(MatRef X, BOp op, int slice) {
     if (g.chkr) {
       ProfScope ps(g, DQMC_K_APPLY_B);
-      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, op, slice, nullptr));
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, op, slice, nullptr, g.ngroups));
       return 0;
     }
     // dense: B = eT2 eV, B^-1 = eV^-1 eTinv2, B^dagger = eV eT2^dagger
@@ -721,8 +769,8 @@ struct Ops {
   int effective_greens(MatRef X) {
     if (g.chkr) {
       ProfScope ps(g, DQMC_K_APPLY_B);
-      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, B_EFF_RIGHT, 0, nullptr));
-      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, B_EFF_INV_LEFT, 0, nullptr));
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, B_EFF_RIGHT, 0, nullptr, g.ngroups));
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, X, B_EFF_INV_LEFT, 0, nullptr, g.ngroups));
       return 0;
     }
     int rc;
@@ -734,8 +782,8 @@ struct Ops {
   int effective_greens_ut(MatRef U, MatRef T) {
     if (g.chkr) {
       ProfScope ps(g, DQMC_K_APPLY_B);
-      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, T, B_EFF_RIGHT, 0, nullptr));
-      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, U, B_EFF_INV_LEFT, 0, nullptr));
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, T, B_EFF_RIGHT, 0, nullptr, g.ngroups));
+      LCHK(launch_apply_B_chkr(s(), d(), nc(), g.n, g.N, U, B_EFF_INV_LEFT, 0, nullptr, g.ngroups));
       return 0;
     }
     int rc;
@@ -944,6 +992,7 @@ int batch_create(const dqmc_params* p, const dqmc_operators* ops, int nchains, i
       g.s = q.safe_mult;
       g.nelem = q.slices / q.safe_mult + 1;
       g.chkr = q.checkerboard;
+      g.ngroups = q.checkerboard == 2 ? ops[i].n_groups : 0;
       g.all_checks = q.all_checks != 0;
       b->groups.push_back(g);
     } else {

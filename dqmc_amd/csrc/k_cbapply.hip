@@ -8,11 +8,15 @@
 // within one flavour block for the hoppings, a diagonal for mu), so a tile of
 // vectors is loaded into LDS once, all five factors are applied in LDS, and the
 // tile is written back: one HBM read and one write of the operand in total.
+// The generic checkerboard (hoppings_checkerboard_generic.jl, slice_matrices.jl:233-393) has
+// G groups of disjoint bonds instead of two groups of plaquettes; its hopping factors are 2x2
+// mixes on a bond's two sites, B = H_G^(1/2) ... H_2^(1/2) H_1 H_2^(1/2) ... H_G^(1/2) mu eV,
+// the same stage pattern with G groups.
 #include "chain.hpp"
 
 namespace dqmc {
 
-enum StageKind : int { ST_EV = 0, ST_MU = 1, ST_MUINV = 2, ST_PLAQ = 3 };
+enum StageKind : int { ST_EV = 0, ST_MU = 1, ST_MUINV = 2, ST_PLAQ = 3, ST_BOND = 4 };
 enum Mode : int { MODE_N = 0, MODE_T = 1, MODE_C = 2 };
 // plaquette variants in ChainDesc::plaq
 enum PlaqVar : int { PV_HALF = 0, PV_HALF_INV = 1, PV_FULL = 2, PV_FULL_INV = 3 };
@@ -22,7 +26,7 @@ struct Stage {
   double power;
 };
 struct StageList {
-  Stage st[5];
+  Stage st[2 * kMaxBondGroups + 1];
   int count;
 };
 
@@ -102,6 +106,47 @@ __device__ inline void plaq_stage(cplx* vec, const cplx* P0, const cplx* Puni, c
   }
 }
 
+// y = op(P) x for a 2x2 row-major P (a bond's two sites)
+template <int MODE>
+__device__ inline void mix2(const cplx* P, cplx x[2], cplx y[2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    cplx acc = mk(0, 0);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (MODE == MODE_N)
+        cfma(acc, P[a * 2 + k], x[k]);
+      else if (MODE == MODE_T)
+        cfma(acc, P[k * 2 + a], x[k]);
+      else
+        cfma_cj(acc, P[k * 2 + a], x[k]);
+    }
+    y[a] = acc;
+  }
+}
+
+// one group of the generic checkerboard on the tile: bonds q0 .. q1-1, blocks B0 [q][4][2][2]
+template <int MODE>
+__device__ inline void bond_stage(cplx* vec, const cplx* B0, const int* bsites, int q0, int q1, int n, int N,
+                                  int nv) {
+  const int nq = q1 - q0;
+  for (int idx = threadIdx.x; idx < nv * 4 * nq; idx += blockDim.x) {
+    const int v = idx / (4 * nq);
+    const int rem = idx - v * 4 * nq;
+    const int b = rem / nq, q = q0 + (rem - b * nq);
+    const gcplx* Pg = gp(B0) + ((size_t)q * 4 + b) * 4;
+    cplx P[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) P[e] = ld(Pg + e);
+    const int s0 = gp(bsites)[2 * q], s1 = gp(bsites)[2 * q + 1];
+    cplx* base = vec + (size_t)v * n + b * N;
+    cplx x[2] = {base[s0], base[s1]}, y[2];
+    mix2<MODE>(P, x, y);
+    base[s0] = y[0];
+    base[s1] = y[1];
+  }
+}
+
 // C, S, R per site for power +1 (interactions.jl:52-58); the power sign is applied later.
 __device__ inline void ev_coeffs(const ChainDesc& c, int l, int i, double& C, cplx& S, double& R) {
   const double* f = c.phi + 3 * ((size_t)i + (size_t)c.N * l);
@@ -133,7 +178,7 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
   const bool uni = c.plaq_uniform != 0;
   const int nplaq_all = N / 4;
   if (uni) {
-    for (int e = threadIdx.x; e < sl.count * 64; e += blockDim.x) {
+    for (int e = threadIdx.x; e < min(sl.count, 5) * 64; e += blockDim.x) {
       const int s = e >> 6, b = (e >> 4) & 3, k = e & 15;
       const Stage st = sl.st[s];
       if (st.kind == ST_PLAQ)
@@ -180,6 +225,12 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
       if (st.mode == MODE_N) ev_stage<MODE_N>(vec, evc, st.power, n, N, nv);
       else if (st.mode == MODE_T) ev_stage<MODE_T>(vec, evc, st.power, n, N, nv);
       else ev_stage<MODE_C>(vec, evc, st.power, n, N, nv);
+    } else if (st.kind == ST_BOND) {
+      const cplx* B0 = c.bond_blocks + (size_t)st.var * c.nbonds * 16;
+      const int q0 = c.grp_off[st.group], q1 = c.grp_off[st.group + 1];
+      if (st.mode == MODE_N) bond_stage<MODE_N>(vec, B0, c.bond_sites, q0, q1, n, N, nv);
+      else if (st.mode == MODE_T) bond_stage<MODE_T>(vec, B0, c.bond_sites, q0, q1, n, N, nv);
+      else bond_stage<MODE_C>(vec, B0, c.bond_sites, q0, q1, n, N, nv);
     } else {  // ST_PLAQ
       const cplx* P0 = c.plaq + (((size_t)st.var * 2 + st.group) * 4) * nplaq * 16;
       const int* sites = c.plaq_sites + (size_t)st.group * nplaq * 4;
@@ -239,61 +290,63 @@ static hipError_t run_stages(hipStream_t s, const ChainDesc* chains, int nchains
   return hipGetLastError();
 }
 
-// Checkerboard factor order (slice_matrices.jl:101-226): groups are [A, B] = [0, 1];
-// B = hop_half[B] hop[A] hop_half[B] mu eV.
+// Checkerboard factor order (slice_matrices.jl:101-226 Assaad, :233-393 generic). With the
+// groups numbered 0 .. G-1 (Assaad: [A, B] = [0, 1]; generic: the greedy bond groups, group 0
+// taking the full step):
+//   B = h[G-1] ... h[1] H[0] h[1] ... h[G-1] mu eV        (h = half step, H = full step)
+// Left application runs the factors right to left on columns, right application left to
+// right on rows (transposed mixes); B^-1 and B^dagger follow by inverting / adjoining each.
 hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
-                               BOp op, int slice, const int* slices) {
+                               BOp op, int slice, const int* slices, int ngroups) {
+  const bool generic = ngroups > 0;
+  const int G = generic ? ngroups : 2;
+  const int kind = generic ? ST_BOND : ST_PLAQ;
+  if (G > kMaxBondGroups) return hipErrorInvalidValue;
   StageList sl;
-  sl.count = 5;
+  sl.count = 0;
   bool rows = false;
+  auto push = [&](int k, int mode, int var, int group, double power) { sl.st[sl.count++] = mkst(k, mode, var, group, power); };
+  // the hopping part h[G-1] ... h[1] H[0] h[1] ... h[G-1] in application order (its two halves
+  // are mirror images, so the order is the same from either side)
+  auto hops = [&](int mode, int half, int full) {
+    for (int g = G - 1; g >= 1; --g) push(kind, mode, half, g, 0);
+    push(kind, mode, full, 0, 0);
+    for (int g = 1; g < G; ++g) push(kind, mode, half, g, 0);
+  };
   switch (op) {
     case B_LEFT:
-      sl.st[0] = mkst(ST_EV, MODE_N, 0, 0, 1.0);
-      sl.st[1] = mkst(ST_MU, MODE_N, 0, 0, 0);
-      sl.st[2] = mkst(ST_PLAQ, MODE_N, PV_HALF, 1, 0);
-      sl.st[3] = mkst(ST_PLAQ, MODE_N, PV_FULL, 0, 0);
-      sl.st[4] = mkst(ST_PLAQ, MODE_N, PV_HALF, 1, 0);
+      push(ST_EV, MODE_N, 0, 0, 1.0);
+      push(ST_MU, MODE_N, 0, 0, 0);
+      hops(MODE_N, PV_HALF, PV_FULL);
       break;
     case B_INV_LEFT:
-      sl.st[0] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 1, 0);
-      sl.st[1] = mkst(ST_PLAQ, MODE_N, PV_FULL_INV, 0, 0);
-      sl.st[2] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 1, 0);
-      sl.st[3] = mkst(ST_MUINV, MODE_N, 0, 0, 0);
-      sl.st[4] = mkst(ST_EV, MODE_N, 0, 0, -1.0);
+      hops(MODE_N, PV_HALF_INV, PV_FULL_INV);
+      push(ST_MUINV, MODE_N, 0, 0, 0);
+      push(ST_EV, MODE_N, 0, 0, -1.0);
       break;
     case B_DAG_LEFT:
-      sl.st[0] = mkst(ST_PLAQ, MODE_C, PV_HALF, 1, 0);
-      sl.st[1] = mkst(ST_PLAQ, MODE_C, PV_FULL, 0, 0);
-      sl.st[2] = mkst(ST_PLAQ, MODE_C, PV_HALF, 1, 0);
-      sl.st[3] = mkst(ST_MU, MODE_N, 0, 0, 0);
-      sl.st[4] = mkst(ST_EV, MODE_C, 0, 0, 1.0);
+      hops(MODE_C, PV_HALF, PV_FULL);
+      push(ST_MU, MODE_N, 0, 0, 0);
+      push(ST_EV, MODE_C, 0, 0, 1.0);
       break;
     case B_RIGHT:
       rows = true;
-      sl.st[0] = mkst(ST_PLAQ, MODE_T, PV_HALF, 1, 0);
-      sl.st[1] = mkst(ST_PLAQ, MODE_T, PV_FULL, 0, 0);
-      sl.st[2] = mkst(ST_PLAQ, MODE_T, PV_HALF, 1, 0);
-      sl.st[3] = mkst(ST_MU, MODE_N, 0, 0, 0);
-      sl.st[4] = mkst(ST_EV, MODE_T, 0, 0, 1.0);
+      hops(MODE_T, PV_HALF, PV_FULL);
+      push(ST_MU, MODE_N, 0, 0, 0);
+      push(ST_EV, MODE_T, 0, 0, 1.0);
       break;
     case B_INV_RIGHT:
       rows = true;
-      sl.st[0] = mkst(ST_EV, MODE_T, 0, 0, -1.0);
-      sl.st[1] = mkst(ST_MUINV, MODE_N, 0, 0, 0);
-      sl.st[2] = mkst(ST_PLAQ, MODE_T, PV_HALF_INV, 1, 0);
-      sl.st[3] = mkst(ST_PLAQ, MODE_T, PV_FULL_INV, 0, 0);
-      sl.st[4] = mkst(ST_PLAQ, MODE_T, PV_HALF_INV, 1, 0);
+      push(ST_EV, MODE_T, 0, 0, -1.0);
+      push(ST_MUINV, MODE_N, 0, 0, 0);
+      hops(MODE_T, PV_HALF_INV, PV_FULL_INV);
       break;
-    case B_EFF_RIGHT:  // X hop_half[B] hop_half[A] (fermion_measurements.jl:1132-1135)
+    case B_EFF_RIGHT:  // X h[G-1] ... h[0] (fermion_measurements.jl:1132-1135)
       rows = true;
-      sl.count = 2;
-      sl.st[0] = mkst(ST_PLAQ, MODE_T, PV_HALF, 1, 0);
-      sl.st[1] = mkst(ST_PLAQ, MODE_T, PV_HALF, 0, 0);
+      for (int g = G - 1; g >= 0; --g) push(kind, MODE_T, PV_HALF, g, 0);
       break;
-    case B_EFF_INV_LEFT:  // hop_half_inv[A] hop_half_inv[B] X (fermion_measurements.jl:1136-1139)
-      sl.count = 2;
-      sl.st[0] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 1, 0);
-      sl.st[1] = mkst(ST_PLAQ, MODE_N, PV_HALF_INV, 0, 0);
+    case B_EFF_INV_LEFT:  // h_inv[0] ... h_inv[G-1] X (fermion_measurements.jl:1136-1139)
+      for (int g = G - 1; g >= 0; --g) push(kind, MODE_N, PV_HALF_INV, g, 0);
       break;
     default:
       return hipErrorInvalidValue;

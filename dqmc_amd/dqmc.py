@@ -63,6 +63,17 @@ def _operators_struct(ops, keep):
         s.plaq_sites = iptr(ps)
         s.plaq = pl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         s.expmu = dptr(em)
+    if ops.group_offsets is not None:
+        go = np.ascontiguousarray(ops.group_offsets, dtype=np.int32)
+        bs = np.ascontiguousarray(ops.bond_sites, dtype=np.int32)
+        bb = np.ascontiguousarray(ops.bond_blocks, dtype=np.complex128)
+        em = np.ascontiguousarray(ops.expmu, dtype=np.float64)
+        keep += [go, bs, bb, em]
+        s.n_groups = len(go) - 1
+        s.group_offsets = iptr(go)
+        s.bond_sites = iptr(bs)
+        s.bond_blocks = bb.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        s.expmu = dptr(em)
     if ops.hop_exp is not None:
         he = np.asfortranarray(ops.hop_exp, dtype=np.complex128)
         hi = np.asfortranarray(ops.hop_exp_inv, dtype=np.complex128)
@@ -73,7 +84,8 @@ def _operators_struct(ops, keep):
 
 
 def _params_struct(p):
-    return _capi.Params(L=p.L, slices=p.slices, safe_mult=p.safe_mult, checkerboard=int(p.checkerboard),
+    return _capi.Params(L=p.L, slices=p.slices, safe_mult=p.safe_mult,
+                        checkerboard=(2 if p.generic else 1) if p.checkerboard else 0,
                         edrun=int(p.edrun), all_checks=int(p.all_checks), delta_tau=p.delta_tau, lam=p.lam,
                         r=p.r, c=p.c, u=p.u)
 

@@ -31,6 +31,7 @@ class Params:
     c: float = 3.0
     u: float = 1.0
     checkerboard: bool = True
+    generic_chkr: bool = False  # DQMC(p; force_generic_chkr=true) (dqmc_framework.jl:105-115)
     bfield: bool = False
     box: float = 0.5
     box_global: float = 0.5
@@ -54,6 +55,11 @@ class Params:
     @property
     def beta(self):
         return self.slices * self.delta_tau
+
+    @property
+    def generic(self):
+        """CBGeneric: the checkerboard at odd L, or when forced (dqmc_framework.jl:105-115)."""
+        return bool(self.checkerboard) and (bool(self.generic_chkr) or self.L % 2 == 1)
 
     @property
     def t(self):
@@ -258,6 +264,63 @@ def _plaquette_blocks(p, prefac, sign):
     return sites, mats
 
 
+def square_bonds(L):
+    """Edges of the square lattice file (lattices/gen_square_lattice_xml.jl): for every site
+    its up bond, then its right bond; (src, trg, vertical), 0-based."""
+    nb = neighbor_table(L)
+    out = []
+    for s in range(L * L):
+        out.append((s, int(nb[0, s]), True))
+        out.append((s, int(nb[1, s]), False))
+    return out
+
+
+def bond_groups(N, bonds):
+    """build_checkerboard (hoppings_checkerboard_generic.jl:1-31): repeated passes over the
+    bonds in file order; a pass takes every unused bond whose sites are still free in it."""
+    used = [False] * len(bonds)
+    groups = []
+    while not all(used):
+        free = [True] * N
+        grp = []
+        for q, (a, b, _) in enumerate(bonds):
+            if used[q] or not free[a] or not free[b]:
+                continue
+            used[q] = True
+            free[a] = free[b] = False
+            grp.append(q)
+        groups.append(grp)
+    return groups
+
+
+def _generic_blocks(p):
+    """(group_offsets, bond_sites, bond_blocks) of the generic checkerboard: each bond's 2x2
+    block of exp(-/+ prefac dtau T_g) (hoppings_checkerboard_generic.jl:41-104, 199-270) for the
+    variants (half, half_inv, full, full_inv) and flavour blocks (s,f) = (1,1),(2,2),(2,1),(1,2).
+    A group's bonds are disjoint, so exp(T_g) is the product of the bonds' 2x2 exponentials."""
+    bonds = square_bonds(p.L)
+    groups = bond_groups(p.N, bonds)
+    order = [q for g in groups for q in g]
+    offs = np.zeros(len(groups) + 1, dtype=np.int32)
+    offs[1:] = np.cumsum([len(g) for g in groups])
+    sites = np.array([[bonds[q][0], bonds[q][1]] for q in order], dtype=np.int32)
+    t = p.t
+    ph = peierls_phases(p) if p.bfield else None
+    blocks = np.zeros((4, len(order), 4, 2, 2), dtype=np.complex128)
+    for k, q in enumerate(order):
+        src, trg, vert = bonds[q]
+        for b, (s_, f) in enumerate(((0, 0), (1, 1), (1, 0), (0, 1))):
+            amp = t[1, f] if vert else t[0, f]
+            T = np.zeros((2, 2), dtype=np.complex128)  # local order (src, trg)
+            T[1, 0] = -amp * (np.exp(1j * ph[s_][f][trg, src]) if p.bfield else 1.0)
+            T[0, 1] = -amp * (np.exp(1j * ph[s_][f][src, trg]) if p.bfield else 1.0)
+            for v, (prefac, sign) in enumerate(((0.5, 1.0), (0.5, -1.0), (1.0, 1.0), (1.0, -1.0))):
+                E = expm(-sign * prefac * p.delta_tau * T)
+                E[np.abs(E) < 1e-15] = 0  # rem_eff_zeros!
+                blocks[v, k, b] = E
+    return offs, sites, blocks
+
+
 @dataclass
 class OperatorSet:
     neighbors: np.ndarray
@@ -266,15 +329,20 @@ class OperatorSet:
     expmu: np.ndarray = None       # [n]
     hop_exp: np.ndarray = None     # [n, n] complex (Fortran order)
     hop_exp_inv: np.ndarray = None
+    group_offsets: np.ndarray = None  # generic checkerboard: [G + 1] int32
+    bond_sites: np.ndarray = None     # [nbonds, 2] int32
+    bond_blocks: np.ndarray = None    # [4 variants, nbonds, 4 blocks, 2, 2] complex128
 
 
 def build_operators(p):
     """Setup-time operators for one chain (init_hopping_matrices, hoppings.jl:1-14)."""
     ops = OperatorSet(neighbors=neighbor_table(p.L))
     N = p.N
-    if p.checkerboard:
-        if p.L % 2:
-            raise ValueError("the Assaad checkerboard needs an even L")
+    if p.generic:
+        ops.group_offsets, ops.bond_sites, ops.bond_blocks = _generic_blocks(p)
+        muv = np.concatenate([np.full(N, p.mu1), np.full(N, p.mu2), np.full(N, p.mu1), np.full(N, p.mu2)])
+        ops.expmu = np.exp(-p.delta_tau * -muv)
+    elif p.checkerboard:
         variants = []
         sites = None
         for prefac, sign in ((0.5, 1.0), (0.5, -1.0), (1.0, 1.0), (1.0, -1.0)):

@@ -39,7 +39,8 @@ typedef struct {
   int L;
   int slices;       /* M = beta / delta_tau */
   int safe_mult;    /* stabilisation interval; slices % safe_mult == 0 */
-  int checkerboard; /* 1: Assaad 4-site checkerboard (CBAssaad), 0: dense (CBFalse) */
+  int checkerboard; /* 1: Assaad 4-site checkerboard (CBAssaad), 2: generic bond checkerboard
+                       (CBGeneric, any L), 0: dense (CBFalse) */
   int edrun;        /* 1: bosonic action has only the mass term (parameters.jl:50) */
   int all_checks;   /* 1: record max|G_wrapped - G_fresh| at each recompute */
   double delta_tau, lambda, r, c, u;
@@ -59,7 +60,15 @@ typedef struct {
  * expmu[4N] is the diagonal of chkr_mu = exp(+dtau mu).
  * Dense (checkerboard = 0): hop_exp / hop_exp_inv are hopping_matrix_exp(_inv)
  * as n x n complex column-major. neighbors[k][i] (k = up, right, down, left)
- * is the reference's neighbor table (lattice.jl:105-117) transposed, 0-based. */
+ * is the reference's neighbor table (lattice.jl:105-117) transposed, 0-based.
+ * Generic checkerboard (checkerboard = 2; init_checkerboard_matrices(_Bfield),
+ * hoppings_checkerboard_generic.jl:1-104, 199-270): l.groups as n_groups groups of disjoint
+ * bonds, group g holding bonds group_offsets[g] .. group_offsets[g+1]-1 (group 0 takes the
+ * full step, l.groups[1]); bond_sites[q][2] = (src, trg) of l.checkerboard[1:2, q];
+ * bond_blocks[v][q][b][r][c] (complex, interleaved) = entry (r, c), r, c in {src, trg}, of the
+ * bond's 2x2 block of variant v (chkr_hop_half, _half_inv, chkr_hop, _inv) in flavour block b;
+ * expmu as above. The folding of chkr groups (fold_chkr_grps) is a regrouping of the same
+ * product and needs no input. */
 typedef struct {
   const int* plaq_sites;
   const double* plaq;
@@ -67,6 +76,10 @@ typedef struct {
   const double* hop_exp;
   const double* hop_exp_inv;
   const int* neighbors;
+  int n_groups;
+  const int* group_offsets;
+  const int* bond_sites;
+  const double* bond_blocks;
 } dqmc_operators;
 
 /* Counters and diagnostics of a chain. */

@@ -22,7 +22,7 @@ class Params:
 
     def __init__(self, L, slices, delta_tau=0.1, safe_mult=10, hoppings=(1.0, 0.5, -0.5, -1.0),
                  mu1=-0.5, mu2=None, lam=1.0, r=1.0, c=3.0, u=1.0, chkr=True, bfield=False,
-                 box=0.5, all_checks=True, edrun=False):
+                 box=0.5, all_checks=True, edrun=False, generic=False, sparsity_limit=0.01):
         self.L = int(L)
         self.slices = int(slices)
         self.delta_tau = float(delta_tau)
@@ -38,6 +38,9 @@ class Params:
         self.c = float(c)
         self.u = float(u)
         self.chkr = bool(chkr)
+        # DQMC(p; force_generic_chkr) (dqmc_framework.jl:105-115): CBGeneric for odd L or on request
+        self.generic = self.chkr and (bool(generic) or self.L % 2 == 1)
+        self.sparsity_limit = float(sparsity_limit)  # parameters.jl:70
         self.bfield = bool(bfield)
         self.box = float(box)
         self.all_checks = bool(all_checks)
@@ -279,6 +282,115 @@ def plaquettes_to_sparse(p, blocks):
     return sp.csc_matrix((np.array(vals, dtype=dtype), (rows, cols)), shape=(n, n))
 
 
+# ----------------------------------------------------------------------------
+# Generic checkerboard (hoppings_checkerboard_generic.jl:1-270)
+# ----------------------------------------------------------------------------
+
+
+def square_bonds(L):
+    """Edges of the square lattice file as lattices/gen_square_lattice_xml.jl writes them:
+    for every site its up bond, then its right bond; (src, trg, vector), 0-based."""
+    nb = neighbors(L)
+    out = []
+    for s in range(L * L):
+        out.append((s, int(nb[0, s]), (0.0, 1.0)))
+        out.append((s, int(nb[1, s]), (1.0, 0.0)))
+    return out
+
+
+def build_checkerboard(N, bonds):
+    """Greedy bond groups (build_checkerboard, hoppings_checkerboard_generic.jl:1-31): repeated
+    passes over the bonds in file order, each taking every unused bond whose two sites are
+    still free in this pass. Returns a list of groups, each a list of bond ids."""
+    used = [False] * len(bonds)
+    groups = []
+    while not all(used):
+        sites_used = [False] * N
+        grp = []
+        for bid, (src, trg, _) in enumerate(bonds):
+            if used[bid] or sites_used[src] or sites_used[trg]:
+                continue
+            used[bid] = True
+            sites_used[src] = sites_used[trg] = True
+            grp.append(bid)
+        groups.append(grp)
+    return groups
+
+
+def _rem_eff_zeros(X):
+    X = np.array(X)
+    X[np.abs(X) < 1e-15] = 0
+    return X
+
+
+def generic_group_exps(p, bonds, groups):
+    """Per group g: the N x N hopping matrix T_g of its bonds for every (spin, flavour) and the
+    four exponentials exp(-dtau/2 T), exp(+dtau/2 T), exp(-dtau T), exp(+dtau T)
+    (init_checkerboard_matrices(_Bfield), hoppings_checkerboard_generic.jl:41-104, 199-270),
+    assembled over the flavour blocks (s,f) = (1,1), (2,2), (2,1), (1,2)."""
+    N = p.N
+    ph = peirls_phases(p) if p.bfield else None
+    out = {k: [] for k in ("half", "half_inv", "full", "full_inv")}
+    for grp in groups:
+        mats = {k: [] for k in out}
+        for (s_, f) in ((0, 0), (1, 1), (1, 0), (0, 1)):
+            T = np.zeros((N, N), dtype=np.complex128 if p.bfield else np.float64)
+            for bid in grp:
+                src, trg, v = bonds[bid]
+                t = p.t[1, f] if v == (0.0, 1.0) else p.t[0, f]  # U: vertical, R: horizontal
+                if p.bfield:
+                    T[trg, src] += -np.exp(1j * ph[s_][f][trg, src]) * t
+                    T[src, trg] += -np.exp(1j * ph[s_][f][src, trg]) * t
+                else:
+                    T[src, trg] += -t
+                    T[trg, src] = T[src, trg]
+            dt = p.delta_tau
+            mats["half"].append(_rem_eff_zeros(expm(-0.5 * dt * T)))
+            mats["half_inv"].append(_rem_eff_zeros(expm(0.5 * dt * T)))
+            mats["full"].append(_rem_eff_zeros(expm(-dt * T)))
+            mats["full_inv"].append(_rem_eff_zeros(expm(dt * T)))
+        for k in out:
+            out[k].append(sp.block_diag(mats[k], format="csc"))
+    return out
+
+
+def fold_chkr_grps(p, hop_half, hop_half_inv, hop_half_dagger):
+    """fold_chkr_grps (hoppings_checkerboard_generic.jl:107-176): groups 2..G (1-based) merged
+    into products until a product's density exceeds p.sparsity_limit."""
+    n = p.n
+    eye = sp.identity(n, dtype=np.complex128, format="csc")
+    G = len(hop_half)
+    folded_mats, folded = [], []
+    cur = eye
+    fstart, fstop = 1, 0  # 0-based group indices
+    for i in range(1, G):
+        if i != 1 and cur.count_nonzero() / float(n * n) > p.sparsity_limit:
+            folded_mats.append(cur)
+            folded.append(range(fstart, fstop + 1))
+            fstart = fstop = i
+            cur = eye
+        else:
+            fstop += 1
+        cur = (hop_half[i] @ cur).tocsc()
+    folded_mats.append(cur)
+    folded.append(range(fstart, fstop + 1))
+    res = {"F": folded_mats, "F_rev": [], "F_inv": [], "F_inv_rev": [], "F_dag": [], "F_dag_rev": []}
+    for rng in folded:
+        c_rev, c_inv, c_inv_rev, c_dag, c_dag_rev = eye, eye, eye, eye, eye
+        for k in rng:
+            c_rev = (c_rev @ hop_half[k]).tocsc()
+            c_inv = (hop_half_inv[k] @ c_inv).tocsc()
+            c_inv_rev = (c_inv_rev @ hop_half_inv[k]).tocsc()
+            c_dag = (hop_half_dagger[k] @ c_dag).tocsc()
+            c_dag_rev = (c_dag_rev @ hop_half_dagger[k]).tocsc()
+        res["F_rev"].append(c_rev)
+        res["F_inv"].append(c_inv)
+        res["F_inv_rev"].append(c_inv_rev)
+        res["F_dag"].append(c_dag)
+        res["F_dag_rev"].append(c_dag_rev)
+    return res
+
+
 class Operators:
     """Host-side operators built once at setup (hoppings.jl, hoppings_checkerboard.jl)."""
 
@@ -292,7 +404,19 @@ class Operators:
         self.chkr_mu_inv = sp.diags(np.exp(dt * -muv)).tocsc()
         self.chkr_mu_half = sp.diags(np.exp(-0.5 * dt * -muv)).tocsc()
         self.chkr_mu_half_inv = sp.diags(np.exp(0.5 * dt * -muv)).tocsc()
-        if p.chkr:
+        self.generic = p.generic
+        if p.generic:
+            self.bonds = square_bonds(p.L)
+            self.groups = build_checkerboard(N, self.bonds)
+            ex = generic_group_exps(p, self.bonds, self.groups)
+            self.chkr_hop_half = ex["half"]
+            self.chkr_hop_half_inv = ex["half_inv"]
+            self.chkr_hop = ex["full"]
+            self.chkr_hop_inv = ex["full_inv"]
+            self.chkr_hop_half_dagger = [m.conj().T.tocsc() for m in self.chkr_hop_half]
+            self.chkr_hop_dagger = [m.conj().T.tocsc() for m in self.chkr_hop]
+            self.fold = fold_chkr_grps(p, self.chkr_hop_half, self.chkr_hop_half_inv, self.chkr_hop_half_dagger)
+        elif p.chkr:
             self.plaq_half = checkerboard_plaquettes(p, 0.5, False)
             self.plaq_half_inv = checkerboard_plaquettes(p, 0.5, True)
             self.plaq = checkerboard_plaquettes(p, 1.0, False)
@@ -384,6 +508,14 @@ class Model:
             return self._slice_matrix_dense(l, 1.0) @ M
         o = self.ops
         eV = interaction_matrix_exp(self.p, self.phi, l, 1.0)
+        if o.generic:  # slice_matrices.jl:236-270
+            M = np.asarray(o.chkr_mu @ (eV @ M))
+            for F in reversed(o.fold["F_rev"]):
+                M = np.asarray(F @ M)
+            M = np.asarray(o.chkr_hop[0] @ M)
+            for F in o.fold["F"]:
+                M = np.asarray(F @ M)
+            return M
         M = eV @ M
         M = o.chkr_mu @ M
         M = o.chkr_hop_half[1] @ M
@@ -397,6 +529,14 @@ class Model:
             return M @ self._slice_matrix_dense(l, 1.0)
         o = self.ops
         eV = interaction_matrix_exp(self.p, self.phi, l, 1.0)
+        if o.generic:  # slice_matrices.jl:272-301
+            for F in reversed(o.fold["F"]):
+                M = _dense_times_sparse(M, F)
+            M = _dense_times_sparse(M, o.chkr_hop[0])
+            for F in o.fold["F_rev"]:
+                M = _dense_times_sparse(M, F)
+            M = _dense_times_sparse(M, o.chkr_mu)
+            return _dense_times_sparse(M, eV)
         M = _dense_times_sparse(M, o.chkr_hop_half[1])
         M = _dense_times_sparse(M, o.chkr_hop[0])
         M = _dense_times_sparse(M, o.chkr_hop_half[1])
@@ -410,6 +550,13 @@ class Model:
             return self._slice_matrix_dense(l, -1.0) @ M
         o = self.ops
         eV = interaction_matrix_exp(self.p, self.phi, l, -1.0)
+        if o.generic:  # slice_matrices.jl:303-332
+            for F in reversed(o.fold["F_inv_rev"]):
+                M = np.asarray(F @ M)
+            M = np.asarray(o.chkr_hop_inv[0] @ M)
+            for F in o.fold["F_inv"]:
+                M = np.asarray(F @ M)
+            return np.asarray(eV @ (o.chkr_mu_inv @ M))
         M = o.chkr_hop_half_inv[1] @ M
         M = o.chkr_hop_inv[0] @ M
         M = o.chkr_hop_half_inv[1] @ M
@@ -425,6 +572,13 @@ class Model:
         eV = interaction_matrix_exp(self.p, self.phi, l, -1.0)
         M = _dense_times_sparse(M, eV)
         M = _dense_times_sparse(M, o.chkr_mu_inv)
+        if o.generic:  # slice_matrices.jl:334-362
+            for F in reversed(o.fold["F_inv"]):
+                M = _dense_times_sparse(M, F)
+            M = _dense_times_sparse(M, o.chkr_hop_inv[0])
+            for F in o.fold["F_inv_rev"]:
+                M = _dense_times_sparse(M, F)
+            return M
         M = _dense_times_sparse(M, o.chkr_hop_half_inv[1])
         M = _dense_times_sparse(M, o.chkr_hop_inv[0])
         M = _dense_times_sparse(M, o.chkr_hop_half_inv[1])
@@ -436,6 +590,13 @@ class Model:
             return self._slice_matrix_dense(l, 1.0).conj().T @ M
         o = self.ops
         eV = interaction_matrix_exp(self.p, self.phi, l, 1.0)
+        if o.generic:  # slice_matrices.jl:364-393
+            for F in reversed(o.fold["F_dag_rev"]):
+                M = np.asarray(F @ M)
+            M = np.asarray(o.chkr_hop_dagger[0] @ M)
+            for F in o.fold["F_dag"]:
+                M = np.asarray(F @ M)
+            return np.asarray(eV @ (o.chkr_mu @ M))
         M = o.chkr_hop_half_dagger[1] @ M
         M = o.chkr_hop_dagger[0] @ M
         M = o.chkr_hop_half_dagger[1] @ M

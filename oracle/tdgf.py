@@ -25,9 +25,9 @@ def effective_greens2greens(m, G):
     (fermion_measurements.jl:1125-1141); G <- eT^-1 G eT without it (:1178-1185)."""
     o = m.ops
     if m.p.chkr:
-        for i in (1, 0):
+        for i in reversed(range(len(o.chkr_hop_half))):
             G = _dense_times_sparse(G, o.chkr_hop_half[i])
-        for i in (1, 0):
+        for i in reversed(range(len(o.chkr_hop_half))):
             G = np.asarray(o.chkr_hop_half_inv[i] @ G)
         return G
     tmp = G @ o.hopping_matrix_exp
@@ -39,9 +39,9 @@ def effective_greens2greens_ut(m, U, T):
     :1187-1193)."""
     o = m.ops
     if m.p.chkr:
-        for i in (1, 0):
+        for i in reversed(range(len(o.chkr_hop_half))):
             T = _dense_times_sparse(T, o.chkr_hop_half[i])
-        for i in (1, 0):
+        for i in reversed(range(len(o.chkr_hop_half))):
             U = np.asarray(o.chkr_hop_half_inv[i] @ U)
         return U, T
     return o.hopping_matrix_exp_inv @ U, T @ o.hopping_matrix_exp

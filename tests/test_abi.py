@@ -42,9 +42,9 @@ def test_library_exports_every_header_symbol(lib):
 
 
 def test_struct_layouts_match_header():
-    # dqmc_params: 6 ints + 5 doubles; dqmc_operators: 6 pointers
+    # dqmc_params: 6 ints + 5 doubles; dqmc_operators: 6 pointers, an int (padded), 3 pointers
     assert ctypes.sizeof(_capi.Params) == 6 * 4 + 5 * 8
-    assert ctypes.sizeof(_capi.Operators) == 6 * 8
+    assert ctypes.sizeof(_capi.Operators) == 10 * 8
 
 
 def test_invalid_arguments_fail_loudly(lib):
@@ -136,3 +136,26 @@ def test_fixture_manifest_hashes():
         assert hashlib.sha256(data).hexdigest() == e["sha256"], e["name"]
     assert len(man) >= 50
     assert ld("greens").shape == (64, 64)
+
+
+@pytest.mark.parametrize("L,bfield,force", [(5, False, False), (3, True, False), (4, False, True)])
+def test_generic_checkerboard_operators_match_oracle(L, bfield, force):
+    """The host's generic-checkerboard operators (bond groups and 2x2 blocks handed to the
+    library) assemble into the oracle's chkr_hop_half / _inv / chkr_hop / _inv matrices."""
+    p = PM.Params(L=L, slices=10, hoppings=(1.0, 0.5, 0.5, 1.0), checkerboard=True, bfield=bfield,
+                  generic_chkr=force)
+    assert p.generic
+    ops = PM.build_operators(p)
+    op = O.Operators(O.Params(L, 10, hoppings=(1.0, 0.5, 0.5, 1.0), chkr=True, bfield=bfield, generic=force))
+    G = len(ops.group_offsets) - 1
+    assert G == len(op.groups)
+    N, n = p.N, p.n
+    for v, ref in enumerate((op.chkr_hop_half, op.chkr_hop_half_inv, op.chkr_hop, op.chkr_hop_inv)):
+        for g in range(G):
+            D = np.eye(n, dtype=np.complex128)
+            for q in range(ops.group_offsets[g], ops.group_offsets[g + 1]):
+                a, b = ops.bond_sites[q]
+                for blk in range(4):
+                    idx = [blk * N + a, blk * N + b]
+                    D[np.ix_(idx, idx)] = ops.bond_blocks[v, q, blk]
+            assert np.abs(D - ref[g].toarray()).max() < 1e-14, (v, g)

@@ -46,7 +46,7 @@ def config_params(amd_mod, L, slices, checkerboard=True, lam=1.0, r=1.0, safe_mu
 def oracle_params(p):
     return O.Params(p.L, p.slices, delta_tau=p.delta_tau, safe_mult=p.safe_mult, hoppings=p.hoppings,
                     mu1=p.mu1, mu2=p.mu2, lam=p.lam, r=p.r, c=p.c, u=p.u, chkr=p.checkerboard, bfield=p.bfield,
-                    box=p.box, all_checks=p.all_checks, edrun=p.edrun)
+                    box=p.box, all_checks=p.all_checks, edrun=p.edrun, generic=p.generic)
 
 
 def rand_c(rng, n):
