@@ -113,6 +113,7 @@ SIGNATURES = {
     "dqmc_batch_get_greens": [_P, _i, _D],
     "dqmc_batch_get_stats": [_P, _i, ctypes.POINTER(Stats)],
     "dqmc_batch_use_graph": [_P, _i],
+    "dqmc_batch_set_merged": [_P, _i],
     "dqmc_use_graph": [_P, _i],
     "dqmc_set_qr_panel": [_P, _i],
     "dqmc_batch_set_qr_panel": [_P, _i],

@@ -37,13 +37,16 @@ enum MatId : int {
 // Named length-n real vectors.
 enum VecId : int { V_DL = 0, V_DR, V_D1, V_D2, V_D3, V_D4, V_D5, V_D6, V_DG, V_ONES, V_QRD, kNumVec };
 
-// MK_XMAT / VK_XVEC address the time-displaced Green's function storage (ChainDesc::xmat)
-enum MatKind : int { MK_NAMED = 0, MK_USTACK = 1, MK_TSTACK = 2, MK_XMAT = 3 };
+// MK_XMAT / VK_XVEC address the time-displaced Green's function storage (ChainDesc::xmat).
+// MK_PLAN / VK_PLAN look the reference up in the chain's own plan table (ChainDesc::plan,
+// entry idx): chains advanced together in one launch may then address different stack
+// elements or named matrices, or none (entry -1: the kernel skips that chain).
+enum MatKind : int { MK_NAMED = 0, MK_USTACK = 1, MK_TSTACK = 2, MK_XMAT = 3, MK_PLAN = 4 };
 struct MatRef {
   int kind;
   int idx;
 };
-enum VecKind : int { VK_NAMED = 0, VK_DSTACK = 1, VK_XVEC = 2 };
+enum VecKind : int { VK_NAMED = 0, VK_DSTACK = 1, VK_XVEC = 2, VK_PLAN = 4 };
 struct VecRef {
   int kind;
   int idx;
@@ -106,7 +109,27 @@ struct ChainDesc {
   unsigned int* bar_count;
   unsigned int* bar_gen;
   int* error;
+  // per-step plan of a merged (lockstep) sweep: [steps][kPlanRoles] encoded references
+  // (kind << 24 | idx, or -1), see MK_PLAN
+  const int* plan;
 };
+
+// Roles of a plan row: the references one update step of a merged sweep needs per chain.
+// PRE: the stack element loaded before it is rebuilt (left or right operand); ADD: source
+// and destination elements of add_slice_sequence_left/right; P1/P2: the loads after it
+// (P2 only at the turning points, after the element was reset to the identity); ID: the
+// element reset to the identity; GW: the copy of G the drift check compares with (or none:
+// no copy, no drift recorded).
+enum PlanRole : int {
+  PR_PRE_DU, PR_PRE_DT, PR_PRE_DD, PR_PRE_SU, PR_PRE_ST, PR_PRE_SD,
+  PR_ADD_SU, PR_ADD_SD, PR_ADD_ST, PR_ADD_DU, PR_ADD_DD, PR_ADD_DT,
+  PR_P1_DU, PR_P1_DT, PR_P1_DD, PR_P1_SU, PR_P1_ST, PR_P1_SD,
+  PR_ID_U, PR_ID_T, PR_ID_D,
+  PR_P2_DU, PR_P2_DT, PR_P2_DD, PR_P2_SU, PR_P2_ST, PR_P2_SD,
+  PR_GW,
+  kPlanRoles
+};
+__host__ __device__ inline int plan_enc(int kind, int idx) { return (kind << 24) | idx; }
 
 // Per-chain scalar slots (ChainDesc::scal).
 enum ScalarId : int {
@@ -133,12 +156,22 @@ enum ScalarId : int {
 
 __device__ inline cplx* matptr(const ChainDesc& c, MatRef r) {
   const size_t nn = (size_t)c.n * c.n;
+  if (r.kind == MK_PLAN) {
+    const int e = c.plan[r.idx];
+    if (e < 0) return nullptr;
+    r = MatRef{e >> 24, e & 0xffffff};
+  }
   if (r.kind == MK_NAMED) return c.mat[r.idx];
   if (r.kind == MK_USTACK) return c.ustack + nn * r.idx;
   if (r.kind == MK_XMAT) return c.xmat + nn * r.idx;
   return c.tstack + nn * r.idx;
 }
 __device__ inline double* vecptr(const ChainDesc& c, VecRef r) {
+  if (r.kind == VK_PLAN) {
+    const int e = c.plan[r.idx];
+    if (e < 0) return nullptr;
+    r = VecRef{e >> 24, e & 0xffffff};
+  }
   if (r.kind == VK_NAMED) return c.vec[r.idx];
   if (r.kind == VK_XVEC) return c.xvec + (size_t)c.n * r.idx;
   return c.dstack + (size_t)c.n * r.idx;
@@ -167,6 +200,10 @@ enum BOp : int { B_LEFT = 0, B_RIGHT, B_INV_LEFT, B_INV_RIGHT, B_DAG_LEFT,
 // the Assaad checkerboard (plaquettes); > 0: the generic one with that many bond groups.
 hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
                                BOp op, int slice, const int* slices, int ngroups = 0);
+// The same with a per-chain choice between two operators of the same side (both left or
+// both right): sel[k] = 0 applies op0, 1 op1, -1 leaves chain k unchanged; slices[k] per chain.
+hipError_t launch_apply_B_chkr_sel(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                                   BOp op0, BOp op1, const int* sel, const int* slices, int ngroups);
 // only the eV factor: mode 0 N, 1 T, 2 C; rows = apply to rows (from the right)
 hipError_t launch_apply_eV(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
                            bool rows, int mode, double power, int slice, const int* slices);

@@ -39,6 +39,12 @@ int fail(int code, const std::string& msg) {
       return fail(DQMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));                     \
   } while (0)
 
+#define RC(x)            \
+  do {                   \
+    int _rc = (x);       \
+    if (_rc) return _rc; \
+  } while (0)
+
 constexpr int kMaxWG = 128;
 
 MatRef NM(int id) { return MatRef{MK_NAMED, id}; }
@@ -134,6 +140,15 @@ struct dqmc_batch {
   std::vector<int> chain_group;  // chain -> group index
   double* d_scratch = nullptr;   // small host-visible staging (detratio etc.)
   bool use_graph = false;
+  // merged sweeps of several shape groups (chains of any L and M, one launch per kernel)
+  bool merged = true;
+  hipStream_t mstream = nullptr;
+  int* d_mplan = nullptr;        // per-chain plan rows, then per-launch selector / slice rows
+  size_t mplan_cap = 0;          // ints
+  ChainDesc* d_mdesc = nullptr;  // descriptors of the active sets of a merged call
+  size_t mdesc_cap = 0;          // descriptors
+  std::vector<int> h_rows, h_sel, h_sli, h_loc;  // host side of the plan (kept until copied)
+  std::vector<dqmc::ChainDesc> h_mdesc;
 };
 
 struct dqmc_ctx {
@@ -952,6 +967,7 @@ int check_device_error(dqmc_batch* b) {
 
 int batch_sync(dqmc_batch* b) {
   for (auto& g : b->groups) HIPCHK(hipStreamSynchronize(g.stream));
+  if (b->mstream) HIPCHK(hipStreamSynchronize(b->mstream));
   return check_device_error(b);
 }
 
@@ -1007,6 +1023,7 @@ int batch_create(const dqmc_params* p, const dqmc_operators* ops, int nchains, i
     int rc = upload_desc(b, g);
     if (rc) return rc;
   }
+  if (b->groups.size() > 1) HIPCHK(hipStreamCreateWithFlags(&b->mstream, hipStreamNonBlocking));
   HIPCHK(hipMalloc(&b->d_scratch, sizeof(double) * 128));
   *out = b;
   return 0;
@@ -1032,6 +1049,9 @@ int batch_destroy(dqmc_batch* b) {
   }
   for (auto& cm : b->chains) chain_free(cm);
   if (b->d_scratch) (void)hipFree(b->d_scratch);
+  if (b->mstream) (void)hipStreamDestroy(b->mstream);
+  if (b->d_mplan) (void)hipFree(b->d_mplan);
+  if (b->d_mdesc) (void)hipFree(b->d_mdesc);
   delete b;
   return 0;
 }
@@ -1145,6 +1165,348 @@ int group_sweep_graph(Group& g, double box, int dolog) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Merged sweeps: every shape group of a batch advanced in lockstep, one launch per kernel for
+// all chains. Each group still follows its own propagate state machine (stack.jl:391-499):
+// with M a multiple of safe_mult s, step j (1-based, counted from init or from the start of a
+// sweep call) recomputes G exactly when j % s == 0 for every chain, whatever its M, so the
+// groups always need the same kind of step (a wrap, or a stack rebuild). Where they differ --
+// which stack element is rebuilt or loaded, which slice matrices apply, whether a turning
+// point resets an element or wraps after the recompute -- each chain reads its own plan row
+// (ChainDesc::plan, MK_PLAN references) and per-launch selector / slice rows. A chain that
+// has done its sweeps leaves the launches (compacted descriptor arrays per active set).
+// ---------------------------------------------------------------------------
+enum StepKind : int { SK_WRAP_UP = 0, SK_WRAP_DOWN, SK_UR, SK_DR, SK_BT, SK_TT };
+
+// What one propagate() of a group does (the host twin of Ops::propagate); advances g.cur/dir.
+struct StepRec {
+  int kind;
+  int cur_new;   // current_slice after the step (local updates at cur_new - 1)
+  int wslice;    // 0-based slice of the wrap of a wrap step, of UR's drift-check wrap, of DR/TT's final wrap
+  int a, b;      // add_slice_sequence: source element a, rebuilt element b
+  bool left;     // add_slice_sequence_left (B) or _right (B^dagger)
+  int pre, post;  // element loaded before the rebuild (UR, DR), element loaded after it
+};
+
+int step_record(Group& g, StepRec& r) {
+  const int M = g.M, s = g.s;
+  if (g.dir == 1) {
+    if (g.cur % s == 0) {
+      if (g.cur == 0) return fail(DQMC_E_STATE, "merged sweep: unexpected state (slice 0, up)");
+      const int cn = g.cur + 1;
+      if (cn <= M) {
+        const int idx = (cn - 1) / s;
+        r = StepRec{SK_UR, cn, cn - 2, idx - 1, idx, true, idx, idx};
+        g.cur = cn;
+        return 0;
+      }
+      r = StepRec{SK_TT, M, M - 1, g.nelem - 2, g.nelem - 1, true, -1, g.nelem - 1};
+      g.cur = M;
+      g.dir = -1;
+      return 0;
+    }
+    r = StepRec{SK_WRAP_UP, g.cur + 1, g.cur - 1, 0, 0, true, -1, -1};
+    g.cur += 1;
+    return 0;
+  }
+  if ((g.cur - 1) % s == 0) {
+    const int cn = g.cur - 1;
+    if (cn == M) return fail(DQMC_E_STATE, "merged sweep: unexpected state (slice M + 1, down)");
+    if (cn > 0) {
+      const int idx = cn / s + 1;
+      r = StepRec{SK_DR, cn, cn - 1, idx, idx - 1, false, idx - 1, idx - 1};
+      g.cur = cn;
+      return 0;
+    }
+    r = StepRec{SK_BT, 1, -1, 1, 0, false, -1, 0};
+    g.cur = 1;
+    g.dir = 1;
+    return 0;
+  }
+  r = StepRec{SK_WRAP_DOWN, g.cur - 1, g.cur - 2, 0, 0, true, -1, -1};
+  g.cur -= 1;
+  return 0;
+}
+
+// plan-row roles of one recompute step for a chain of group g with record r
+void plan_row(const Group& g, const StepRec& r, int* row) {
+  for (int k = 0; k < kPlanRoles; ++k) row[k] = -1;
+  const int U = MK_USTACK, T = MK_TSTACK, D = VK_DSTACK, NMD = MK_NAMED, NVD = VK_NAMED;
+  auto load = [&](int base, int elem, bool to_left) {
+    row[base + 0] = plan_enc(NMD, to_left ? M_UL : M_UR);
+    row[base + 1] = plan_enc(NMD, to_left ? M_TL : M_TR);
+    row[base + 2] = plan_enc(NVD, to_left ? V_DL : V_DR);
+    row[base + 3] = plan_enc(U, elem);
+    row[base + 4] = plan_enc(T, elem);
+    row[base + 5] = plan_enc(D, elem);
+  };
+  if (r.kind == SK_UR) load(PR_PRE_DU, r.pre, false);
+  if (r.kind == SK_DR) load(PR_PRE_DU, r.pre, true);
+  row[PR_ADD_SU] = plan_enc(U, r.a);
+  row[PR_ADD_SD] = plan_enc(D, r.a);
+  row[PR_ADD_ST] = plan_enc(T, r.a);
+  row[PR_ADD_DU] = plan_enc(U, r.b);
+  row[PR_ADD_DD] = plan_enc(D, r.b);
+  row[PR_ADD_DT] = plan_enc(T, r.b);
+  const bool post_left = r.kind == SK_UR || r.kind == SK_TT;
+  load(PR_P1_DU, r.post, post_left);
+  if (r.kind == SK_BT || r.kind == SK_TT) {
+    row[PR_ID_U] = plan_enc(U, r.post);
+    row[PR_ID_T] = plan_enc(T, r.post);
+    row[PR_ID_D] = plan_enc(D, r.post);
+    load(PR_P2_DU, r.post, !post_left);
+  }
+  if (g.all_checks && (r.kind == SK_UR || r.kind == SK_DR)) row[PR_GW] = plan_enc(NMD, M_GW);
+}
+
+constexpr int kMergedApplies = 4 + 16;  // apply launches of a step with a selector row: 2 + s + 2, s <= 16
+
+// Enqueue nsweeps sweeps of every chain of the batch (all groups compatible) on b->mstream.
+int merged_sweeps(dqmc_batch* b, int nsweeps, double box, int dolog) {
+  const int ng = (int)b->groups.size();
+  const Group& g0 = b->groups[0];
+  const int s = g0.s;
+  if (s > 16) return fail(DQMC_E_ARG, "merged sweeps need safe_mult <= 16");
+  int Mmax = 0, nmax = 0, Nmax = 0, nmin = 1 << 30;
+  for (auto& g : b->groups) {
+    Mmax = std::max(Mmax, g.M);
+    nmax = std::max(nmax, g.n);
+    Nmax = std::max(Nmax, g.N);
+    nmin = std::min(nmin, g.n);
+  }
+  const long long total = (long long)nsweeps * Mmax;
+  if (total == 0) return 0;
+  const int nch = (int)b->chains.size();
+  const int TC = std::min<long long>(total, Mmax);  // steps per plan chunk
+  // plan buffer: per chain [TC][kPlanRoles]; then [TC][kMergedApplies][nch] selectors and slices,
+  // [TC][nch] local-update slices
+  const size_t n_rows = (size_t)nch * TC * kPlanRoles;
+  const size_t n_sel = (size_t)TC * kMergedApplies * nch;
+  const size_t n_loc = (size_t)TC * nch;
+  const size_t need = n_rows + 2 * n_sel + n_loc;
+  if (need > b->mplan_cap) {
+    HIPCHK(hipStreamSynchronize(b->mstream));
+    if (b->d_mplan) HIPCHK(hipFree(b->d_mplan));
+    b->d_mplan = nullptr;
+    HIPCHK(hipMalloc(&b->d_mplan, need * sizeof(int)));
+    b->mplan_cap = need;
+  }
+  int* d_rows = b->d_mplan;
+  int* d_sel = d_rows + n_rows;
+  int* d_sli = d_sel + n_sel;
+  int* d_loc = d_sli + n_sel;
+  // active sets: chain of group g active while t < nsweeps * g.M; one descriptor array each
+  std::vector<int> Ms;
+  for (auto& g : b->groups) Ms.push_back(g.M);
+  std::sort(Ms.begin(), Ms.end());
+  Ms.erase(std::unique(Ms.begin(), Ms.end()), Ms.end());
+  std::vector<std::vector<int>> sets;  // chain indices, set k = chains with M >= Ms[k]
+  for (int Mk : Ms) {
+    std::vector<int> v;
+    for (int ci = 0; ci < nch; ++ci)
+      if (b->chains[ci].M >= Mk) v.push_back(ci);
+    sets.push_back(v);
+  }
+  size_t ndesc = 0;
+  for (auto& v : sets) ndesc += v.size();
+  if (ndesc > b->mdesc_cap) {
+    HIPCHK(hipStreamSynchronize(b->mstream));
+    if (b->d_mdesc) HIPCHK(hipFree(b->d_mdesc));
+    b->d_mdesc = nullptr;
+    HIPCHK(hipMalloc(&b->d_mdesc, ndesc * sizeof(ChainDesc)));
+    b->mdesc_cap = ndesc;
+  }
+  std::vector<size_t> set_off(sets.size());
+  {
+    HIPCHK(hipStreamSynchronize(b->mstream));  // the previous call's descriptor copy is done
+    std::vector<ChainDesc>& h = b->h_mdesc;
+    h.clear();
+    for (size_t k = 0; k < sets.size(); ++k) {
+      set_off[k] = h.size();
+      for (int ci : sets[k]) {
+        ChainDesc d = b->chains[ci].d;
+        d.plan = d_rows + (size_t)ci * TC * kPlanRoles;
+        h.push_back(d);
+      }
+    }
+    // the merged stream waits for the groups' queued work
+    for (auto& g : b->groups) {
+      hipEvent_t e = g.take_event();
+      HIPCHK(hipEventRecord(e, g.stream));
+      HIPCHK(hipStreamWaitEvent(b->mstream, e, 0));
+      g.event_pool.push_back(e);
+    }
+    HIPCHK(hipMemcpyAsync(b->d_mdesc, h.data(), h.size() * sizeof(ChainDesc), hipMemcpyHostToDevice, b->mstream));
+  }
+  // a pseudo group for the Ops helpers: the merged stream, the current active set
+  Group mg;
+  mg.n = nmax;
+  mg.N = Nmax;
+  mg.L = 0;
+  mg.M = Mmax;
+  mg.s = s;
+  mg.nelem = 0;
+  mg.chkr = g0.chkr;
+  mg.ngroups = g0.ngroups;
+  mg.all_checks = g0.all_checks;
+  mg.stream = b->mstream;
+  mg.greens_route = g0.greens_route;
+  mg.udt_route = g0.udt_route;
+  mg.qr_nb = g0.qr_nb ? g0.qr_nb : 8;
+  mg.wg_inv = g0.wg_inv;
+  mg.local_variant = g0.local_variant;
+  std::vector<int>& hrows = b->h_rows;
+  std::vector<int>& hsel = b->h_sel;
+  std::vector<int>& hsli = b->h_sli;
+  std::vector<int>& hloc = b->h_loc;
+  std::vector<StepRec> rec(ng);
+  for (long long t0 = 0; t0 < total; t0 += TC) {
+    const int T = (int)std::min<long long>(TC, total - t0);
+    // the host rows are copied asynchronously: the previous chunk's copies must be done
+    HIPCHK(hipStreamSynchronize(b->mstream));
+    hrows.resize(n_rows);
+    hsel.resize(n_sel);
+    hsli.resize(n_sel);
+    hloc.resize(n_loc);
+    std::fill(hrows.begin(), hrows.end(), -1);
+    std::fill(hsel.begin(), hsel.end(), -1);
+    std::fill(hsli.begin(), hsli.end(), 0);
+    std::fill(hloc.begin(), hloc.end(), 0);
+    std::vector<int> kinds(T), set_of(T);
+    for (int t = 0; t < T; ++t) {
+      const long long ts = t0 + t;
+      int sk = 0;
+      while (sk < (int)Ms.size() && ts >= (long long)nsweeps * Ms[sk]) ++sk;
+      set_of[t] = sk;
+      const std::vector<int>& act = sets[sk];
+      // records of the active groups (every chain of a group shares its record)
+      int kind_cls = -1;
+      for (int gi = 0; gi < ng; ++gi) {
+        Group& g = b->groups[gi];
+        if (ts >= (long long)nsweeps * g.M) continue;
+        RC(step_record(g, rec[gi]));
+        const int cls = rec[gi].kind <= SK_WRAP_DOWN ? 0 : 1;
+        if (kind_cls < 0) kind_cls = cls;
+        if (cls != kind_cls) return fail(DQMC_E_STATE, "merged sweep: groups out of step (safe_mult / slices)");
+      }
+      kinds[t] = kind_cls;
+      for (int k = 0; k < (int)act.size(); ++k) {
+        const int ci = act[k];
+        const Group& g = b->groups[b->chain_group[ci]];
+        const StepRec& r = rec[b->chain_group[ci]];
+        int* row = hrows.data() + ((size_t)ci * TC + t) * kPlanRoles;
+        int* sel = hsel.data() + (size_t)t * kMergedApplies * nch;
+        int* sli = hsli.data() + (size_t)t * kMergedApplies * nch;
+        hloc[(size_t)t * nch + k] = r.cur_new - 1;
+        auto set_apply = [&](int a, int which, int slice) {
+          sel[(size_t)a * nch + k] = which;
+          sli[(size_t)a * nch + k] = slice;
+        };
+        if (kind_cls == 0) {
+          const int w = r.kind == SK_WRAP_UP ? 0 : 1;
+          set_apply(0, w, r.wslice);
+          set_apply(1, w, r.wslice);
+          continue;
+        }
+        plan_row(g, r, row);
+        if (r.kind == SK_UR && g.all_checks) {  // drift check: wrap(GW, cur_new - 1, +1)
+          set_apply(0, 0, r.wslice);
+          set_apply(1, 0, r.wslice);
+        }
+        for (int j = 0; j < s; ++j)  // add_slice_sequence_left: B_l ascending; _right: B_l^dagger descending
+          set_apply(2 + j, r.left ? 0 : 1, r.left ? r.a * s + j : r.a * s - 1 - j);
+        if (r.kind == SK_DR || r.kind == SK_TT) {  // wrap(G, cur_new + 1, -1)
+          set_apply(2 + s, 0, r.wslice);
+          set_apply(3 + s, 0, r.wslice);
+        }
+      }
+    }
+    HIPCHK(hipMemcpyAsync(d_rows, hrows.data(), n_rows * sizeof(int), hipMemcpyHostToDevice, b->mstream));
+    HIPCHK(hipMemcpyAsync(d_sel, hsel.data(), n_sel * sizeof(int), hipMemcpyHostToDevice, b->mstream));
+    HIPCHK(hipMemcpyAsync(d_sli, hsli.data(), n_sel * sizeof(int), hipMemcpyHostToDevice, b->mstream));
+    HIPCHK(hipMemcpyAsync(d_loc, hloc.data(), n_loc * sizeof(int), hipMemcpyHostToDevice, b->mstream));
+    for (int t = 0; t < T; ++t) {
+      const std::vector<int>& act = sets[set_of[t]];
+      const int na = (int)act.size();
+      int an = 0, aN = 0, amin = 1 << 30;
+      for (int ci : act) {
+        an = std::max(an, b->chains[ci].n);
+        aN = std::max(aN, b->chains[ci].N);
+        amin = std::min(amin, b->chains[ci].n);
+      }
+      mg.d_desc = b->d_mdesc + set_off[set_of[t]];
+      mg.chains.assign(na, 0);
+      mg.n = an;
+      mg.N = aN;
+      // one workgroup per CU over all active chains
+      const int per = std::max(1, 256 / na);
+      mg.wg_qr = std::max(1, std::min({per, amin / mg.qr_nb, kMaxWG}));
+      mg.wg_local = std::max(1, std::min({per, amin / 4, kMaxWG}));
+      if (local_window_sites(an, mg.wg_local) < 2) mg.wg_local = std::max(1, std::min({per, amin / 8, kMaxWG}));
+      Ops o(mg);
+      const ChainDesc* dd = mg.d_desc;
+      const int* sel = d_sel + (size_t)t * kMergedApplies * nch;
+      const int* sli = d_sli + (size_t)t * kMergedApplies * nch;
+      auto apply = [&](MatRef X, BOp op0, BOp op1, int a) -> int {
+        ProfScope ps(mg, DQMC_K_APPLY_B);
+        LCHK(launch_apply_B_chkr_sel(mg.stream, dd, na, an, aN, X, op0, op1, sel + (size_t)a * nch,
+                                     sli + (size_t)a * nch, mg.ngroups));
+        return 0;
+      };
+      auto P = [&](int role) { return MatRef{MK_PLAN, t * kPlanRoles + role}; };
+      auto PV = [&](int role) { return VecRef{VK_PLAN, t * kPlanRoles + role}; };
+      if (kinds[t] == 0) {
+        RC(apply(NM(M_G), B_LEFT, B_INV_LEFT, 0));
+        RC(apply(NM(M_G), B_INV_RIGHT, B_RIGHT, 1));
+      } else {
+        RC(o.copy(P(PR_PRE_DU), P(PR_PRE_SU)));
+        RC(o.copy(P(PR_PRE_DT), P(PR_PRE_ST)));
+        RC(o.copyv(PV(PR_PRE_DD), PV(PR_PRE_SD)));
+        if (mg.all_checks) {
+          RC(o.copy(P(PR_GW), NM(M_G)));
+          RC(apply(NM(M_GW), B_LEFT, B_LEFT, 0));
+          RC(apply(NM(M_GW), B_INV_RIGHT, B_INV_RIGHT, 1));
+        }
+        RC(o.copy(NM(M_CURU), P(PR_ADD_SU)));
+        for (int j = 0; j < s; ++j) RC(apply(NM(M_CURU), B_LEFT, B_DAG_LEFT, 2 + j));
+        LCHK(launch_scale_cols(mg.stream, dd, na, an, NM(M_CURU), PV(PR_ADD_SD)));
+        RC(o.udt(NM(M_CURU), PV(PR_ADD_DD), P(PR_ADD_DU), NM(M_QRT), true));
+        RC(o.gemm(P(PR_ADD_DT), NM(M_QRT), OP_N, P(PR_ADD_ST), OP_N));
+        RC(o.copy(P(PR_P1_DU), P(PR_P1_SU)));
+        RC(o.copy(P(PR_P1_DT), P(PR_P1_ST)));
+        RC(o.copyv(PV(PR_P1_DD), PV(PR_P1_SD)));
+        LCHK(launch_identity(mg.stream, dd, na, an, P(PR_ID_U), NOVEC));
+        LCHK(launch_identity(mg.stream, dd, na, an, P(PR_ID_T), NOVEC));
+        LCHK(launch_ones_vec(mg.stream, dd, na, an, PV(PR_ID_D)));
+        RC(o.copy(P(PR_P2_DU), P(PR_P2_SU)));
+        RC(o.copy(P(PR_P2_DT), P(PR_P2_ST)));
+        RC(o.copyv(PV(PR_P2_DD), PV(PR_P2_SD)));
+        RC(o.calculate_greens(true));
+        if (mg.all_checks) LCHK(launch_maxdiff(mg.stream, dd, na, an, P(PR_GW), NM(M_G)));
+        RC(apply(NM(M_G), B_INV_LEFT, B_INV_LEFT, 2 + s));
+        RC(apply(NM(M_G), B_RIGHT, B_RIGHT, 3 + s));
+      }
+      {
+        ProfScope ps(mg, DQMC_K_LOCAL);
+        int variant = mg.local_variant;
+        if (variant == 0)
+          variant = local_window_sites(an, mg.wg_local) >= 2 ? 4 : local_tile_fits(an, mg.wg_local) ? 2
+                    : local_delayed_fits(an) ? 3 : 1;
+        LCHK(launch_local_updates(mg.stream, dd, na, an, 0, d_loc + (size_t)t * nch, box, mg.wg_local, dolog,
+                                  variant));
+      }
+    }
+  }
+  // the groups' later work waits for the merged stream
+  hipEvent_t e = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(e, b->mstream));
+  for (auto& g : b->groups) HIPCHK(hipStreamWaitEvent(g.stream, e, 0));
+  (void)hipEventDestroy(e);
+  (void)nmin;
+  return 0;
+}
+
 // device addresses of a chain's matrices and vectors, as matptr/vecptr compute them on the device
 cplx* host_matptr(const ChainMem& cm, MatRef r) {
   const size_t nn = (size_t)cm.n * cm.n;
@@ -1204,11 +1566,6 @@ double read_scalar(dqmc_batch* b, int chain, int slot) {
 // ===========================================================================
 #define CTX_CHECK(ctx) \
   if (!(ctx) || !(ctx)->b) return fail(DQMC_E_ARG, "null context")
-#define RC(x)              \
-  do {                     \
-    int _rc = (x);         \
-    if (_rc) return _rc;   \
-  } while (0)
 
 extern "C" {
 
@@ -2063,6 +2420,19 @@ int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box) {
     for (int ci : g.chains)
       if (box[ci] != bx) return fail(DQMC_E_ARG, "chains of one shape group must share the box size");
   }
+  if (b->merged && b->groups.size() > 1) {
+    bool ok = true;
+    const Group& g0 = b->groups[0];
+    for (auto& g : b->groups)
+      ok = ok && g.s == g0.s && g.chkr == g0.chkr && g.ngroups == g0.ngroups && g.all_checks == g0.all_checks &&
+           g.greens_route == 0 && g.udt_route == g0.udt_route && g.local_variant == g0.local_variant &&
+           box[g.chains[0]] == box[g0.chains[0]] && g.cur >= 1 && g.cur <= g.M && g.s <= 16;
+    if (ok) {
+      int dolog = 0;
+      for (auto& cm : b->chains) dolog |= cm.log_cap > 0;
+      return merged_sweeps(b, nsweeps, box[g0.chains[0]], dolog);
+    }
+  }
   bool graphs = b->use_graph;
   for (auto& g : b->groups) graphs = graphs && g.prof_mask == 0;  // timing events stay eager
   if (graphs) {
@@ -2091,6 +2461,12 @@ int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box) {
       more = more || steps[gi] < total;
     }
   }
+  return 0;
+}
+int dqmc_batch_set_merged(dqmc_batch* b, int enable) {
+  if (!b) return fail(DQMC_E_ARG, "null batch");
+  RC(batch_sync(b));
+  b->merged = enable != 0;
   return 0;
 }
 int dqmc_batch_synchronize(dqmc_batch* b) {

@@ -158,16 +158,21 @@ __device__ inline void ev_coeffs(const ChainDesc& c, int l, int i, double& C, cp
   R = -f[2] * sh;
 }
 
+// sel (optional): per chain 0 = stage list sl, 1 = sl1, -1 = leave the chain unchanged
 template <bool ROWS>
-__global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, MatRef Xr, StageList sl, int slice,
-                                                      const int* slices, int vpb) {
+__global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, MatRef Xr, StageList sl0, StageList sl1,
+                                                      const int* sel, int slice, const int* slices, int vpb) {
   const ChainDesc& c = chains[blockIdx.y];
   const int n = c.n, N = c.N;
   const int v0 = blockIdx.x * vpb;
   if (v0 >= n) return;
+  const int which = sel ? sel[blockIdx.y] : 0;
+  if (which < 0) return;
+  const StageList& sl = which == 0 ? sl0 : sl1;
   const int nv = min(vpb, n - v0);
   const int l = slices ? slices[blockIdx.y] : slice;
   cplx* X = matptr(c, Xr);
+  if (!X) return;
 
   extern __shared__ double smem[];
   cplx* vec = reinterpret_cast<cplx*>(smem);          // [vpb][n]
@@ -272,7 +277,9 @@ static Stage mkst(int kind, int mode, int var, int group, double power) {
 }
 
 static hipError_t run_stages(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
-                             bool rows, const StageList& sl, int slice, const int* slices) {
+                             bool rows, const StageList& sl, int slice, const int* slices,
+                             const StageList* sl1 = nullptr, const int* sel = nullptr) {
+  const StageList& s1 = sl1 ? *sl1 : sl;
   // tile of vectors per block: enough blocks to cover the chip, LDS <= 64 KB
   int vpb = rows ? 8 : 4;
   while (vpb > 1 && (size_t)vpb * nmax * 16 > 65536) vpb >>= 1;
@@ -284,9 +291,9 @@ static hipError_t run_stages(hipStream_t s, const ChainDesc* chains, int nchains
     if (e != hipSuccess) return e;
   }
   if (rows)
-    hipLaunchKernelGGL(k_apply_stages<true>, grid, dim3(256), lds, s, chains, X, sl, slice, slices, vpb);
+    hipLaunchKernelGGL(k_apply_stages<true>, grid, dim3(256), lds, s, chains, X, sl, s1, sel, slice, slices, vpb);
   else
-    hipLaunchKernelGGL(k_apply_stages<false>, grid, dim3(256), lds, s, chains, X, sl, slice, slices, vpb);
+    hipLaunchKernelGGL(k_apply_stages<false>, grid, dim3(256), lds, s, chains, X, sl, s1, sel, slice, slices, vpb);
   return hipGetLastError();
 }
 
@@ -296,15 +303,13 @@ static hipError_t run_stages(hipStream_t s, const ChainDesc* chains, int nchains
 //   B = h[G-1] ... h[1] H[0] h[1] ... h[G-1] mu eV        (h = half step, H = full step)
 // Left application runs the factors right to left on columns, right application left to
 // right on rows (transposed mixes); B^-1 and B^dagger follow by inverting / adjoining each.
-hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
-                               BOp op, int slice, const int* slices, int ngroups) {
+static hipError_t stage_list(BOp op, int ngroups, StageList& sl, bool& rows) {
   const bool generic = ngroups > 0;
   const int G = generic ? ngroups : 2;
   const int kind = generic ? ST_BOND : ST_PLAQ;
   if (G > kMaxBondGroups) return hipErrorInvalidValue;
-  StageList sl;
   sl.count = 0;
-  bool rows = false;
+  rows = false;
   auto push = [&](int k, int mode, int var, int group, double power) { sl.st[sl.count++] = mkst(k, mode, var, group, power); };
   // the hopping part h[G-1] ... h[1] H[0] h[1] ... h[G-1] in application order (its two halves
   // are mirror images, so the order is the same from either side)
@@ -351,7 +356,27 @@ hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchai
     default:
       return hipErrorInvalidValue;
   }
+  return hipSuccess;
+}
+
+hipError_t launch_apply_B_chkr(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                               BOp op, int slice, const int* slices, int ngroups) {
+  StageList sl;
+  bool rows = false;
+  hipError_t e = stage_list(op, ngroups, sl, rows);
+  if (e != hipSuccess) return e;
   return run_stages(s, chains, nchains, nmax, Nmax, X, rows, sl, slice, slices);
+}
+
+hipError_t launch_apply_B_chkr_sel(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int Nmax, MatRef X,
+                                   BOp op0, BOp op1, const int* sel, const int* slices, int ngroups) {
+  StageList sl0, sl1;
+  bool rows0 = false, rows1 = false;
+  hipError_t e = stage_list(op0, ngroups, sl0, rows0);
+  if (e == hipSuccess) e = stage_list(op1, ngroups, sl1, rows1);
+  if (e != hipSuccess) return e;
+  if (rows0 != rows1) return hipErrorInvalidValue;
+  return run_stages(s, chains, nchains, nmax, Nmax, X, rows0, sl0, 0, slices, &sl1, sel);
 }
 
 // Only the eV factor (used by the dense no-checkerboard path).

@@ -197,6 +197,7 @@ __global__ __launch_bounds__(256) void k_elementwise(const ChainDesc* chains, in
   const int n = c.n;
   const size_t nn = (size_t)n * n;
   cplx* D = matptr(c, Dr);
+  if (!D) return;  // a merged sweep's plan skips this chain
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t start = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (op == EW_COPY) {
@@ -283,6 +284,7 @@ __global__ __launch_bounds__(256) void k_vec(const ChainDesc* chains, int op, Ve
   const ChainDesc& c = chains[blockIdx.y];
   double* d = vecptr(c, Dr);
   const double* sv = vecptr(c, Sr);
+  if (!d) return;  // a merged sweep's plan skips this chain
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += gridDim.x * blockDim.x) {
     if (op == VO_COPY)
       d[i] = sv[i];
@@ -328,6 +330,7 @@ hipError_t launch_split_scales(hipStream_t s, const ChainDesc* chains, int nchai
 __global__ __launch_bounds__(256) void k_maxdiff(const ChainDesc* chains, MatRef Ar, MatRef Br, int phase) {
   const ChainDesc& c = chains[blockIdx.y];
   unsigned long long* acc = reinterpret_cast<unsigned long long*>(c.scal + S_DRIFT_TMP);
+  if (!matptr(c, Ar)) return;  // a merged sweep's plan records no drift for this chain
   if (phase == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *acc = 0ull;
     return;

@@ -612,6 +612,11 @@ class Batch:
         """Capture each shape group's sweep in a HIP graph and replay it (see DQMC.use_graph)."""
         check(self.lib.dqmc_batch_use_graph(self.h, 1 if enable else 0))
 
+    def set_merged(self, enable=True):
+        """Sweep all shape groups in lockstep, one launch per kernel for every chain (default),
+        or group by group."""
+        check(self.lib.dqmc_batch_set_merged(self.h, 1 if enable else 0))
+
     def autotune(self, reps=3):
         """DQMC.autotune for every shape group; returns one dict per group."""
         box = np.array([p.box for p in self.params])

@@ -307,6 +307,13 @@ int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st);
  * first use and replayed after; changing workgroup counts or routes drops the graphs. Same
  * results as eager launches; not used while kernel timing (dqmc_batch_profile) is on. */
 int dqmc_batch_use_graph(dqmc_batch* b, int enable);
+/* No reference counterpart. 1 (default): a batch of several shape groups (any L and M, same
+ * safe_mult, checkerboard kind, all_checks, box and the default calculate_greens route) is
+ * swept in lockstep, every kernel launched once for all chains: all chains recompute G on the
+ * same steps (M is a multiple of safe_mult), and each chain's own stack elements, slices
+ * and turning points come from a per-chain plan; a chain leaves the launches when its sweeps
+ * are done. Same results as sweeping the groups one by one (0); ignores use_graph. */
+int dqmc_batch_set_merged(dqmc_batch* b, int enable);
 int dqmc_use_graph(dqmc_ctx* ctx, int enable);
 /* Autotuner. No reference counterpart (the reference has one fixed CPU code path). Per
  * shape group: times the local-update kernel over (variant, workgroups per chain) and the

@@ -1,0 +1,87 @@
+"""Merged (lockstep) sweeps of batches with several shapes against group-by-group sweeps and
+the oracle, through the C ABI.
+
+A merged sweep launches every kernel once for all chains of all shapes; each chain still
+follows its own propagate state machine (stack.jl:391-499) through its plan rows. The same
+uniform streams give the same decisions as sweeping the shape groups one by one: fields
+identical, G within 1e-12 (the launches differ only in workgroup counts, i.e. summation
+order), the same acceptance counts, stream positions and stack states.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import dqmc_oracle as O
+from tests.test_gpu_parity import config_params, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print("[gpu-test] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import dqmc_amd
+    dqmc_amd.load()
+    return dqmc_amd
+
+
+def _batch(mod, ps, merged, seed0):
+    from dqmc_amd import Batch, UniformStream
+    b = Batch(ps, streams=[UniformStream(seed0 + k) for k in range(len(ps))])
+    b.set_merged(merged)
+    confs = [np.random.default_rng(seed0 + 50 + k).uniform(-1, 1, (3, p.N, p.slices)) for k, p in enumerate(ps)]
+    b.init(confs)
+    return b, confs
+
+
+def test_merged_equals_groupwise(amd):
+    mod = amd
+    ps = [config_params(mod, 4, 20), config_params(mod, 6, 40, lam=1.5), config_params(mod, 4, 40, r=-1.0),
+          config_params(mod, 6, 20, lam=0.7), config_params(mod, 8, 30, r=0.0), config_params(mod, 4, 20, lam=2.0)]
+    _say("merged vs group-by-group: 6 chains, 5 shapes, 3 + 2 sweeps")
+    bm, _ = _batch(mod, ps, True, 3100)
+    bg, _ = _batch(mod, ps, False, 3100)
+    for b in (bm, bg):
+        b.sweep(3)
+        b.sweep(2, sync=False)
+        b.synchronize()
+    for k, p in enumerate(ps):
+        sm, sg = bm.stats(k), bg.stats(k)
+        assert sm.device_error == 0
+        assert (sm.current_slice, sm.direction) == (sg.current_slice, sg.direction), k
+        assert sm.accepted == sg.accepted and sm.proposed == sg.proposed == 5 * p.N * p.slices, k
+        assert bm.streams[k].consumed == bg.streams[k].consumed, k
+        assert np.array_equal(bm.get_field(k), bg.get_field(k)), k
+        assert np.abs(bm.greens(k) - bg.greens(k)).max() < 1e-12, k
+        assert sm.drift_max < 1e-8 and abs(sm.drift_max - sg.drift_max) < 1e-12, k
+        assert abs(sm.log_det - sg.log_det) < 1e-9 * max(1.0, abs(sg.log_det)), k
+    bm.close()
+    bg.close()
+
+
+def test_merged_chain_matches_oracle(amd):
+    """A chain of a merged batch (L=6, beta=2, with L=4, beta=4 and L=8, beta=3 beside it)
+    reproduces the oracle's field, G and stream consumption after 3 sweeps."""
+    mod = amd
+    ps = [config_params(mod, 4, 40), config_params(mod, 6, 20, lam=1.3), config_params(mod, 8, 30)]
+    _say("merged batch chain vs oracle: L=6 M=20 next to L=4 M=40 and L=8 M=30, 3 sweeps")
+    b, confs = _batch(mod, ps, True, 3200)
+    b.sweep(3)
+    k = 1
+    p = ps[k]
+    o = O.DQMC(oracle_params(p), confs[k])
+    o.init()
+    ostream = O.UniformStream(O.host_uniforms(3200 + k, 8 * p.N * p.slices * 3 + 64))
+    for _ in range(3):
+        o.sweep(ostream)
+    assert np.abs(b.get_field(k) - o.phi).max() < 1e-12
+    assert np.abs(b.greens(k) - o.greens).max() < 1e-10
+    assert b.streams[k].consumed == ostream.pos
+    st = b.stats(k)
+    assert (st.current_slice, st.direction) == (o.current_slice, o.direction)
+    b.close()
