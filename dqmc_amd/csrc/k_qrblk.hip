@@ -407,6 +407,87 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   }
 }
 
+// Column layout of k_qr_solve / k_form_Q_blocked: a workgroup owns CB columns in LDS. With
+// CB <= 4 the four waves split as CB columns x (4 / CB) row slices; with CB > 4 each wave owns
+// CB / 4 columns and all rows, so every V entry read from LDS serves CB / 4 columns.
+template <int CB>
+struct ColLayout {
+  static constexpr int RS = CB >= 4 ? 1 : 4 / CB;   // row slices per column
+  static constexpr int CPW = CB >= 4 ? CB / 4 : 1;  // columns per wave
+};
+
+// x <- (I - V op(T) V^H) x for this wave's CPW columns xc + (jl0 + c) n (rows >= k0); V [NB][n]
+// and T (Tl[j * NB + l] = T(j, l)) in LDS; op = T^H (ADJ, applying Q^H) or T (applying Q).
+// part: LDS for the row-slice sums when RS > 1.
+template <int NB, int CB, bool ADJ>
+__device__ inline void apply_panel_cols(cplx* xc, const cplx* V, const cplx* Tl, cplx* part, int n, int k0, int jl0,
+                                        int rs, int ncol, int lane, int wave) {
+  using Lay = ColLayout<CB>;
+  constexpr int RS = Lay::RS, CPW = Lay::CPW;
+  cplx wv[CPW][NB];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) wv[c][j] = mk(0, 0);
+  for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
+    cplx pj[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const cplx ak = xc[(size_t)(jl0 + c) * n + k];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) cfma_cj(wv[c][j], pj[j], ak);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) wave_sum_many<NB>(wv[c]);
+  if (RS > 1) {
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NB; ++j) part[wave * NB + j] = wv[0][j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      cplx sj = mk(0, 0);
+#pragma unroll
+      for (int q = 0; q < RS; ++q) sj += part[(jl0 + CB * q) * NB + j];
+      wv[0][j] = sj;
+    }
+  }
+  // z = op(T) w: lane j < NB forms z_j of every column, then every lane reads them
+  const int jz = lane & (NB - 1);
+  cplx z[CPW][NB];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    cplx zl = mk(0, 0);
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+      if (ADJ)
+        cfma_cj(zl, Tl[l * NB + jz], wv[c][l]);
+      else
+        cfma(zl, Tl[jz * NB + l], wv[c][l]);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) z[c][j] = read_lane_c(zl, j);
+  }
+  if (jl0 < ncol) {  // wave-uniform; columns past ncol of the group are scratch
+    for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
+      cplx pj[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        cplx ak = xc[(size_t)(jl0 + c) * n + k];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) ak = ak - pj[j] * z[c][j];
+        xc[(size_t)(jl0 + c) * n + k] = ak;
+      }
+    }
+  }
+}
+
 // B <- R^-1 Q^H B for a matrix factored by k_qr_blocked without pre-pivoting (Ar holds R and
 // the reflectors, qr_tau / qr_Tf the rest). Every right-hand side is independent: each
 // workgroup owns CB columns of B in LDS, applies the panels' Q_p^H in order, then
@@ -415,7 +496,7 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
 // inner products are summed over the slices through LDS.
 template <int NB, int CB>
 __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRef Ar, MatRef Br) {
-  constexpr int RS = 4 / CB;
+  using Lay = ColLayout<CB>;
   int bx, by;
   xcd_block(bx, by);
   const ChainDesc& c = chains[by];
@@ -424,7 +505,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
   if (j0 >= n) return;
   const int ncol = min(CB, n - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int jl = wave % CB, rs = wave / CB;
+  const int jl0 = CB >= 4 ? wave * Lay::CPW : wave % CB, rs = CB >= 4 ? 0 : wave / CB;
   const gcplx* A = gp(matptr(c, Ar));
   gcplx* B = gp(matptr(c, Br));
   extern __shared__ double smem[];
@@ -432,7 +513,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
   cplx* V = bc + (size_t)CB * n;              // [NB][n]
   __shared__ cplx Tl[NB * NB];
   __shared__ cplx Rd[16 * 16];
-  __shared__ cplx part[RS > 1 ? 4 * NB : 1];
+  __shared__ cplx part[Lay::RS > 1 ? 4 * NB : 1];
   for (int r = tid; r < n; r += 256) {
     cplx v[CB];
 #pragma unroll
@@ -443,60 +524,13 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
 #pragma unroll
     for (int q = 0; q < CB; ++q) bc[(size_t)q * n + r] = v[q];
   }
-  const bool active = jl < ncol;  // wave-uniform
-  cplx* a = bc + (size_t)jl * n;
   for (int k0 = 0, p = 0; k0 < n; k0 += NB, ++p) {
     const int kb = min(NB, n - k0);
     __syncthreads();
     stage_panel<NB>(V, A, nullptr, k0, kb, n, tid, 1);  // columns >= kb zero-filled
     for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = ld(gp(c.qr_Tf) + (size_t)p * NB * NB + e);
     __syncthreads();
-    // w = V^H a over this wave's row slice (rows k0 + lane + 64 (rs + RS m))
-    cplx wv[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) wv[j] = mk(0, 0);
-    for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
-      const cplx ak = a[k];
-      cplx pj[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], ak);
-    }
-    wave_sum_many<NB>(wv);
-    if (RS > 1) {
-      if (lane == 0) {
-#pragma unroll
-        for (int j = 0; j < NB; ++j) part[wave * NB + j] = wv[j];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        cplx sj = mk(0, 0);
-#pragma unroll
-        for (int q = 0; q < RS; ++q) sj += part[(jl + CB * q) * NB + j];
-        wv[j] = sj;
-      }
-    }
-    // z = T^H w: lane j < NB forms z_j, then every lane reads all of them
-    const int jz = lane & (NB - 1);
-    cplx zl = mk(0, 0);
-#pragma unroll
-    for (int l = 0; l < NB; ++l) cfma_cj(zl, Tl[l * NB + jz], wv[l]);
-    cplx z[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) z[j] = read_lane_c(zl, j);
-    if (active) {
-      for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
-        cplx ak = a[k];
-        cplx pj[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) ak = ak - pj[j] * z[j];
-        a[k] = ak;
-      }
-    }
+    apply_panel_cols<NB, CB, true>(bc, V, Tl, part, n, k0, jl0, rs, ncol, lane, wave);
   }
   // back substitution R y = z in blocks of 16 rows from the bottom: wave q < CB solves the
   // 16 x 16 diagonal block for column q (lane r holds row lo + r), then every thread takes
@@ -510,8 +544,8 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
       Rd[tid] = (kk < bs && rr < bs) ? ld(A + (size_t)(lo + kk) * n + lo + rr) : mk(0, 0);  // Rd[kk*16+rr] = R(lo+rr, lo+kk)
     }
     __syncthreads();
-    if (wave < ncol) {
-      cplx* y = bc + (size_t)wave * n;
+    for (int q = wave; q < ncol; q += 4) {  // wave-uniform: the 16 x 16 diagonal solve of column q
+      cplx* y = bc + (size_t)q * n;
       const int r = lo + (lane & 15);
       cplx zr = (lane < bs) ? y[r] : mk(0, 0);
       for (int k = bs - 1; k >= 0; --k) {
@@ -561,7 +595,7 @@ __global__ __launch_bounds__(256) void k_qr_solve(const ChainDesc* chains, MatRe
 // in k_qr_solve. Reflector k lives in physical column qr_perm[k] below its diagonal.
 template <int NB, int CB>
 __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains, MatRef Ar, MatRef Qr) {
-  constexpr int RS = 4 / CB;
+  using Lay = ColLayout<CB>;
   int bx, by;
   xcd_block(bx, by);
   const ChainDesc& c = chains[by];
@@ -570,22 +604,20 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
   if (j0 >= n) return;
   const int ncol = min(CB, n - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int jl = wave % CB, rs = wave / CB;
+  const int jl0 = CB >= 4 ? wave * Lay::CPW : wave % CB, rs = CB >= 4 ? 0 : wave / CB;
   const gcplx* A = gp(matptr(c, Ar));
   gcplx* Q = gp(matptr(c, Qr));
   extern __shared__ double smem[];
   cplx* xc = reinterpret_cast<cplx*>(smem);  // [CB][n]
   cplx* V = xc + (size_t)CB * n;              // [NB][n]
   __shared__ cplx Tl[NB * NB];
-  __shared__ cplx part[RS > 1 ? 4 * NB : 1];
+  __shared__ cplx part[Lay::RS > 1 ? 4 * NB : 1];
   for (int r = tid; r < n; r += 256)
 #pragma unroll
     for (int q = 0; q < CB; ++q) {
       xc[(size_t)q * n + r].re = (r == j0 + q) ? 1.0 : 0.0;
       xc[(size_t)q * n + r].im = 0.0;
     }
-  const bool active = jl < ncol;  // wave-uniform
-  cplx* a = xc + (size_t)jl * n;
   const int jmax = j0 + ncol - 1;  // panels starting beyond it leave these columns alone
   for (int p = jmax / NB; p >= 0; --p) {
     const int k0 = p * NB, kb = min(NB, n - k0);
@@ -593,51 +625,7 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
     stage_panel<NB>(V, A, c.qr_perm, k0, kb, n, tid, 1);  // columns >= kb zero-filled
     for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = ld(gp(c.qr_Tf) + (size_t)p * NB * NB + e);
     __syncthreads();
-    cplx wv[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) wv[j] = mk(0, 0);
-    for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
-      const cplx ak = a[k];
-      cplx pj[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) cfma_cj(wv[j], pj[j], ak);
-    }
-    wave_sum_many<NB>(wv);
-    if (RS > 1) {
-      if (lane == 0) {
-#pragma unroll
-        for (int j = 0; j < NB; ++j) part[wave * NB + j] = wv[j];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        cplx sj = mk(0, 0);
-#pragma unroll
-        for (int q = 0; q < RS; ++q) sj += part[(jl + CB * q) * NB + j];
-        wv[j] = sj;
-      }
-    }
-    // z = T w: lane j < NB forms z_j, then every lane reads all of them
-    const int jz = lane & (NB - 1);
-    cplx zl = mk(0, 0);
-#pragma unroll
-    for (int l = 0; l < NB; ++l) cfma(zl, Tl[jz * NB + l], wv[l]);
-    cplx z[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) z[j] = read_lane_c(zl, j);
-    if (active) {
-      for (int k = k0 + lane + 64 * rs; k < n; k += 64 * RS) {
-        cplx ak = a[k];
-        cplx pj[NB];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) pj[j] = V[(size_t)j * n + k];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) ak = ak - pj[j] * z[j];
-        a[k] = ak;
-      }
-    }
+    apply_panel_cols<NB, CB, false>(xc, V, Tl, part, n, k0, jl0, rs, ncol, lane, wave);
   }
   __syncthreads();
   for (int r = tid; r < n; r += 256) {
@@ -653,6 +641,11 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
       }
   }
 }
+
+// chain counts from which k_form_Q_blocked uses 8 or 16 columns per workgroup (k_qr_solve
+// keeps 4: its time per workgroup is the latency of the panel sequence, and wider blocks
+// measured slower)
+constexpr int kWideColChains = 8;
 
 // default panel width: 8 (the register-resident panel keeps nb x rows-per-thread values per
 // thread; at nb = 16 the per-column chain is longer and the kernel spills)
@@ -716,6 +709,9 @@ hipError_t launch_form_Q_blocked(hipStream_t s, const ChainDesc* chains, int nch
   if (nb == 0) nb = qr_panel_width(nmax);
   if (nb == 16 && nmax <= 512) return launch_form_Q_blocked_nb<16, 4>(s, chains, nchains, nmax, A, Q);
   if (nb != 8) return hipErrorInvalidValue;
+  // many chains fill the chip with fewer, wider column blocks: each reads the reflectors once
+  if (nchains >= kWideColChains && nmax <= 400) return launch_form_Q_blocked_nb<8, 16>(s, chains, nchains, nmax, A, Q);
+  if (nchains >= kWideColChains && nmax <= 576) return launch_form_Q_blocked_nb<8, 8>(s, chains, nchains, nmax, A, Q);
   if (nmax <= 640) return launch_form_Q_blocked_nb<8, 4>(s, chains, nchains, nmax, A, Q);
   return launch_form_Q_blocked_nb<8, 1>(s, chains, nchains, nmax, A, Q);
 }

@@ -85,3 +85,25 @@ def test_merged_chain_matches_oracle(amd):
     st = b.stats(k)
     assert (st.current_slice, st.direction) == (o.current_slice, o.direction)
     b.close()
+
+
+@pytest.mark.parametrize("L,M", [(6, 20), (12, 10)])
+def test_wide_column_kernels_match_single_chains(amd, L, M):
+    """From 8 chains per launch the Q formation gives each workgroup 16 (n <= 400) or 8 (n <= 576)
+    columns instead of 4; a 9-chain batch must still equal single chains."""
+    mod = amd
+    from dqmc_amd import DQMC, Batch, UniformStream
+    _say("wide-column Q formation: 9 chains L=%d M=%d vs single chains" % (L, M))
+    ps = [config_params(mod, L, M, lam=0.8 + 0.1 * k) for k in range(9)]
+    confs = [np.random.default_rng(3300 + k).uniform(-1, 1, (3, p.N, p.slices)) for k, p in enumerate(ps)]
+    b = Batch(ps, streams=[UniformStream(3400 + k) for k in range(9)])
+    b.init(confs)
+    b.sweep(2)
+    for k in (0, 4, 8):
+        c = DQMC(ps[k], uniforms=UniformStream(3400 + k))
+        c.init(confs[k])
+        c.sweep(2)
+        assert np.array_equal(b.get_field(k), c.hsfield), k
+        assert np.abs(b.greens(k) - c.greens).max() < 1e-12, k
+        c.close()
+    b.close()
