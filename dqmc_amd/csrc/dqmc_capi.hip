@@ -147,8 +147,28 @@ struct dqmc_batch {
   size_t mplan_cap = 0;          // ints
   ChainDesc* d_mdesc = nullptr;  // descriptors of the active sets of a merged call
   size_t mdesc_cap = 0;          // descriptors
-  std::vector<int> h_rows, h_sel, h_sli, h_loc;  // host side of the plan (kept until copied)
+  std::vector<int> h_rows;  // host side of the plan (kept until copied)
   std::vector<dqmc::ChainDesc> h_mdesc;
+  // merged sweeps replayed from HIP graphs, one per start state (dqmc_batch_use_graph)
+  struct MergedGraph {
+    std::vector<int> key, end;  // groups' (current_slice, direction) before / after
+    std::vector<dqmc::ChainDesc> descs;  // the chains' descriptors the graph was captured with
+    double box;
+    int dolog;
+    int* d_plan = nullptr;
+    dqmc::ChainDesc* d_desc = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<MergedGraph> mgraphs;
+  void drop_mgraphs() {
+    if (!mgraphs.empty() && mstream) (void)hipStreamSynchronize(mstream);
+    for (auto& e : mgraphs) {
+      (void)hipGraphExecDestroy(e.exec);
+      (void)hipFree(e.d_plan);
+      (void)hipFree(e.d_desc);
+    }
+    mgraphs.clear();
+  }
 };
 
 struct dqmc_ctx {
@@ -1049,6 +1069,7 @@ int batch_destroy(dqmc_batch* b) {
   }
   for (auto& cm : b->chains) chain_free(cm);
   if (b->d_scratch) (void)hipFree(b->d_scratch);
+  b->drop_mgraphs();
   if (b->mstream) (void)hipStreamDestroy(b->mstream);
   if (b->d_mplan) (void)hipFree(b->d_mplan);
   if (b->d_mdesc) (void)hipFree(b->d_mdesc);
@@ -1261,89 +1282,128 @@ void plan_row(const Group& g, const StepRec& r, int* row) {
 
 constexpr int kMergedApplies = 4 + 16;  // apply launches of a step with a selector row: 2 + s + 2, s <= 16
 
-// Enqueue nsweeps sweeps of every chain of the batch (all groups compatible) on b->mstream.
-int merged_sweeps(dqmc_batch* b, int nsweeps, double box, int dolog) {
-  const int ng = (int)b->groups.size();
-  const Group& g0 = b->groups[0];
-  const int s = g0.s;
-  if (s > 16) return fail(DQMC_E_ARG, "merged sweeps need safe_mult <= 16");
-  int Mmax = 0, nmax = 0, Nmax = 0, nmin = 1 << 30;
+// Shape facts of a merged call and the layout of its plan buffer: per chain [T][kPlanRoles]
+// rows, then [T][kMergedApplies][nch] selectors and slices, then [T][nch] local-update slices.
+struct MergedShape {
+  int s = 0, Mmax = 0, nch = 0;
+  std::vector<int> Ms;                // distinct M, ascending
+  std::vector<std::vector<int>> sets;  // set k: the chains with M >= Ms[k] (active while t < nsweeps Ms[k])
+  size_t ints(int T) const {
+    return (size_t)nch * T * kPlanRoles + 2 * (size_t)T * kMergedApplies * nch + (size_t)T * nch;
+  }
+};
+
+MergedShape merged_shape(dqmc_batch* b) {
+  MergedShape ms;
+  ms.s = b->groups[0].s;
+  ms.nch = (int)b->chains.size();
   for (auto& g : b->groups) {
-    Mmax = std::max(Mmax, g.M);
-    nmax = std::max(nmax, g.n);
-    Nmax = std::max(Nmax, g.N);
-    nmin = std::min(nmin, g.n);
+    ms.Mmax = std::max(ms.Mmax, g.M);
+    ms.Ms.push_back(g.M);
   }
-  const long long total = (long long)nsweeps * Mmax;
-  if (total == 0) return 0;
-  const int nch = (int)b->chains.size();
-  const int TC = std::min<long long>(total, Mmax);  // steps per plan chunk
-  // plan buffer: per chain [TC][kPlanRoles]; then [TC][kMergedApplies][nch] selectors and slices,
-  // [TC][nch] local-update slices
-  const size_t n_rows = (size_t)nch * TC * kPlanRoles;
-  const size_t n_sel = (size_t)TC * kMergedApplies * nch;
-  const size_t n_loc = (size_t)TC * nch;
-  const size_t need = n_rows + 2 * n_sel + n_loc;
-  if (need > b->mplan_cap) {
-    HIPCHK(hipStreamSynchronize(b->mstream));
-    if (b->d_mplan) HIPCHK(hipFree(b->d_mplan));
-    b->d_mplan = nullptr;
-    HIPCHK(hipMalloc(&b->d_mplan, need * sizeof(int)));
-    b->mplan_cap = need;
-  }
-  int* d_rows = b->d_mplan;
-  int* d_sel = d_rows + n_rows;
-  int* d_sli = d_sel + n_sel;
-  int* d_loc = d_sli + n_sel;
-  // active sets: chain of group g active while t < nsweeps * g.M; one descriptor array each
-  std::vector<int> Ms;
-  for (auto& g : b->groups) Ms.push_back(g.M);
-  std::sort(Ms.begin(), Ms.end());
-  Ms.erase(std::unique(Ms.begin(), Ms.end()), Ms.end());
-  std::vector<std::vector<int>> sets;  // chain indices, set k = chains with M >= Ms[k]
-  for (int Mk : Ms) {
+  std::sort(ms.Ms.begin(), ms.Ms.end());
+  ms.Ms.erase(std::unique(ms.Ms.begin(), ms.Ms.end()), ms.Ms.end());
+  for (int Mk : ms.Ms) {
     std::vector<int> v;
-    for (int ci = 0; ci < nch; ++ci)
+    for (int ci = 0; ci < ms.nch; ++ci)
       if (b->chains[ci].M >= Mk) v.push_back(ci);
-    sets.push_back(v);
+    ms.sets.push_back(v);
   }
-  size_t ndesc = 0;
-  for (auto& v : sets) ndesc += v.size();
-  if (ndesc > b->mdesc_cap) {
-    HIPCHK(hipStreamSynchronize(b->mstream));
-    if (b->d_mdesc) HIPCHK(hipFree(b->d_mdesc));
-    b->d_mdesc = nullptr;
-    HIPCHK(hipMalloc(&b->d_mdesc, ndesc * sizeof(ChainDesc)));
-    b->mdesc_cap = ndesc;
-  }
-  std::vector<size_t> set_off(sets.size());
-  {
-    HIPCHK(hipStreamSynchronize(b->mstream));  // the previous call's descriptor copy is done
-    std::vector<ChainDesc>& h = b->h_mdesc;
-    h.clear();
-    for (size_t k = 0; k < sets.size(); ++k) {
-      set_off[k] = h.size();
-      for (int ci : sets[k]) {
-        ChainDesc d = b->chains[ci].d;
-        d.plan = d_rows + (size_t)ci * TC * kPlanRoles;
-        h.push_back(d);
+  return ms;
+}
+
+// The host side of steps t0 .. t0+T-1 of a call of nsweeps sweeps (advances the groups' states):
+// plan ints (layout of MergedShape::ints(T)), the step kind (0 wrap, 1 recompute) and active set.
+int merged_plan(dqmc_batch* b, const MergedShape& ms, int nsweeps, long long t0, int T, std::vector<int>& h,
+                std::vector<int>& kinds, std::vector<int>& set_of) {
+  const int ng = (int)b->groups.size(), nch = ms.nch, s = ms.s;
+  h.assign(ms.ints(T), 0);
+  int* hrows = h.data();
+  int* hsel = hrows + (size_t)nch * T * kPlanRoles;
+  int* hsli = hsel + (size_t)T * kMergedApplies * nch;
+  int* hloc = hsli + (size_t)T * kMergedApplies * nch;
+  std::fill(hrows, hsel, -1);
+  std::fill(hsel, hsli, -1);
+  kinds.assign(T, 0);
+  set_of.assign(T, 0);
+  std::vector<StepRec> rec(ng);
+  for (int t = 0; t < T; ++t) {
+    const long long ts = t0 + t;
+    int sk = 0;
+    while (sk < (int)ms.Ms.size() && ts >= (long long)nsweeps * ms.Ms[sk]) ++sk;
+    set_of[t] = sk;
+    const std::vector<int>& act = ms.sets[sk];
+    int kind_cls = -1;
+    for (int gi = 0; gi < ng; ++gi) {
+      Group& g = b->groups[gi];
+      if (ts >= (long long)nsweeps * g.M) continue;
+      RC(step_record(g, rec[gi]));
+      const int cls = rec[gi].kind <= SK_WRAP_DOWN ? 0 : 1;
+      if (kind_cls < 0) kind_cls = cls;
+      if (cls != kind_cls) return fail(DQMC_E_STATE, "merged sweep: groups out of step (safe_mult / slices)");
+    }
+    kinds[t] = kind_cls;
+    for (int k = 0; k < (int)act.size(); ++k) {
+      const int ci = act[k];
+      const Group& g = b->groups[b->chain_group[ci]];
+      const StepRec& r = rec[b->chain_group[ci]];
+      int* row = hrows + ((size_t)ci * T + t) * kPlanRoles;
+      int* sel = hsel + (size_t)t * kMergedApplies * nch;
+      int* sli = hsli + (size_t)t * kMergedApplies * nch;
+      hloc[(size_t)t * nch + k] = r.cur_new - 1;
+      auto set_apply = [&](int a, int which, int slice) {
+        sel[(size_t)a * nch + k] = which;
+        sli[(size_t)a * nch + k] = slice;
+      };
+      if (kind_cls == 0) {
+        const int w = r.kind == SK_WRAP_UP ? 0 : 1;
+        set_apply(0, w, r.wslice);
+        set_apply(1, w, r.wslice);
+        continue;
+      }
+      plan_row(g, r, row);
+      if (r.kind == SK_UR && g.all_checks) {  // drift check: wrap(GW, cur_new - 1, +1)
+        set_apply(0, 0, r.wslice);
+        set_apply(1, 0, r.wslice);
+      }
+      for (int j = 0; j < s; ++j)  // add_slice_sequence_left: B_l ascending; _right: B_l^dagger descending
+        set_apply(2 + j, r.left ? 0 : 1, r.left ? r.a * s + j : r.a * s - 1 - j);
+      if (r.kind == SK_DR || r.kind == SK_TT) {  // wrap(G, cur_new + 1, -1)
+        set_apply(2 + s, 0, r.wslice);
+        set_apply(3 + s, 0, r.wslice);
       }
     }
-    // the merged stream waits for the groups' queued work
-    for (auto& g : b->groups) {
-      hipEvent_t e = g.take_event();
-      HIPCHK(hipEventRecord(e, g.stream));
-      HIPCHK(hipStreamWaitEvent(b->mstream, e, 0));
-      g.event_pool.push_back(e);
-    }
-    HIPCHK(hipMemcpyAsync(b->d_mdesc, h.data(), h.size() * sizeof(ChainDesc), hipMemcpyHostToDevice, b->mstream));
   }
-  // a pseudo group for the Ops helpers: the merged stream, the current active set
-  Group mg;
-  mg.n = nmax;
-  mg.N = Nmax;
+  return 0;
+}
+
+// Descriptor arrays of the active sets, each chain's pointing to its plan rows at d_plan.
+void merged_descs(dqmc_batch* b, const MergedShape& ms, int T, const int* d_plan, std::vector<ChainDesc>& h,
+                  std::vector<size_t>& set_off) {
+  h.clear();
+  set_off.assign(ms.sets.size(), 0);
+  for (size_t k = 0; k < ms.sets.size(); ++k) {
+    set_off[k] = h.size();
+    for (int ci : ms.sets[k]) {
+      ChainDesc d = b->chains[ci].d;
+      d.plan = d_plan + (size_t)ci * T * kPlanRoles;
+      h.push_back(d);
+    }
+  }
+}
+
+// Enqueue the launches of T planned steps on b->mstream (no host synchronisation).
+int merged_enqueue(dqmc_batch* b, const MergedShape& ms, int T, const int* d_plan, const ChainDesc* d_desc,
+                   const std::vector<size_t>& set_off, const std::vector<int>& kinds, const std::vector<int>& set_of,
+                   double box, int dolog) {
+  const Group& g0 = b->groups[0];
+  const int nch = ms.nch, s = ms.s;
+  const int* d_sel = d_plan + (size_t)nch * T * kPlanRoles;
+  const int* d_sli = d_sel + (size_t)T * kMergedApplies * nch;
+  const int* d_loc = d_sli + (size_t)T * kMergedApplies * nch;
+  Group mg;  // a pseudo group for the Ops helpers: the merged stream, the current active set
   mg.L = 0;
-  mg.M = Mmax;
+  mg.M = ms.Mmax;
   mg.s = s;
   mg.nelem = 0;
   mg.chkr = g0.chkr;
@@ -1355,156 +1415,240 @@ int merged_sweeps(dqmc_batch* b, int nsweeps, double box, int dolog) {
   mg.qr_nb = g0.qr_nb ? g0.qr_nb : 8;
   mg.wg_inv = g0.wg_inv;
   mg.local_variant = g0.local_variant;
-  std::vector<int>& hrows = b->h_rows;
-  std::vector<int>& hsel = b->h_sel;
-  std::vector<int>& hsli = b->h_sli;
-  std::vector<int>& hloc = b->h_loc;
-  std::vector<StepRec> rec(ng);
-  for (long long t0 = 0; t0 < total; t0 += TC) {
-    const int T = (int)std::min<long long>(TC, total - t0);
-    // the host rows are copied asynchronously: the previous chunk's copies must be done
-    HIPCHK(hipStreamSynchronize(b->mstream));
-    hrows.resize(n_rows);
-    hsel.resize(n_sel);
-    hsli.resize(n_sel);
-    hloc.resize(n_loc);
-    std::fill(hrows.begin(), hrows.end(), -1);
-    std::fill(hsel.begin(), hsel.end(), -1);
-    std::fill(hsli.begin(), hsli.end(), 0);
-    std::fill(hloc.begin(), hloc.end(), 0);
-    std::vector<int> kinds(T), set_of(T);
-    for (int t = 0; t < T; ++t) {
-      const long long ts = t0 + t;
-      int sk = 0;
-      while (sk < (int)Ms.size() && ts >= (long long)nsweeps * Ms[sk]) ++sk;
-      set_of[t] = sk;
-      const std::vector<int>& act = sets[sk];
-      // records of the active groups (every chain of a group shares its record)
-      int kind_cls = -1;
-      for (int gi = 0; gi < ng; ++gi) {
-        Group& g = b->groups[gi];
-        if (ts >= (long long)nsweeps * g.M) continue;
-        RC(step_record(g, rec[gi]));
-        const int cls = rec[gi].kind <= SK_WRAP_DOWN ? 0 : 1;
-        if (kind_cls < 0) kind_cls = cls;
-        if (cls != kind_cls) return fail(DQMC_E_STATE, "merged sweep: groups out of step (safe_mult / slices)");
-      }
-      kinds[t] = kind_cls;
-      for (int k = 0; k < (int)act.size(); ++k) {
-        const int ci = act[k];
-        const Group& g = b->groups[b->chain_group[ci]];
-        const StepRec& r = rec[b->chain_group[ci]];
-        int* row = hrows.data() + ((size_t)ci * TC + t) * kPlanRoles;
-        int* sel = hsel.data() + (size_t)t * kMergedApplies * nch;
-        int* sli = hsli.data() + (size_t)t * kMergedApplies * nch;
-        hloc[(size_t)t * nch + k] = r.cur_new - 1;
-        auto set_apply = [&](int a, int which, int slice) {
-          sel[(size_t)a * nch + k] = which;
-          sli[(size_t)a * nch + k] = slice;
-        };
-        if (kind_cls == 0) {
-          const int w = r.kind == SK_WRAP_UP ? 0 : 1;
-          set_apply(0, w, r.wslice);
-          set_apply(1, w, r.wslice);
-          continue;
-        }
-        plan_row(g, r, row);
-        if (r.kind == SK_UR && g.all_checks) {  // drift check: wrap(GW, cur_new - 1, +1)
-          set_apply(0, 0, r.wslice);
-          set_apply(1, 0, r.wslice);
-        }
-        for (int j = 0; j < s; ++j)  // add_slice_sequence_left: B_l ascending; _right: B_l^dagger descending
-          set_apply(2 + j, r.left ? 0 : 1, r.left ? r.a * s + j : r.a * s - 1 - j);
-        if (r.kind == SK_DR || r.kind == SK_TT) {  // wrap(G, cur_new + 1, -1)
-          set_apply(2 + s, 0, r.wslice);
-          set_apply(3 + s, 0, r.wslice);
-        }
-      }
+  mg.prof_mask = g0.prof_mask;
+  for (int t = 0; t < T; ++t) {
+    const std::vector<int>& act = ms.sets[set_of[t]];
+    const int na = (int)act.size();
+    int an = 0, aN = 0, amin = 1 << 30;
+    for (int ci : act) {
+      an = std::max(an, b->chains[ci].n);
+      aN = std::max(aN, b->chains[ci].N);
+      amin = std::min(amin, b->chains[ci].n);
     }
-    HIPCHK(hipMemcpyAsync(d_rows, hrows.data(), n_rows * sizeof(int), hipMemcpyHostToDevice, b->mstream));
-    HIPCHK(hipMemcpyAsync(d_sel, hsel.data(), n_sel * sizeof(int), hipMemcpyHostToDevice, b->mstream));
-    HIPCHK(hipMemcpyAsync(d_sli, hsli.data(), n_sel * sizeof(int), hipMemcpyHostToDevice, b->mstream));
-    HIPCHK(hipMemcpyAsync(d_loc, hloc.data(), n_loc * sizeof(int), hipMemcpyHostToDevice, b->mstream));
-    for (int t = 0; t < T; ++t) {
-      const std::vector<int>& act = sets[set_of[t]];
-      const int na = (int)act.size();
-      int an = 0, aN = 0, amin = 1 << 30;
-      for (int ci : act) {
-        an = std::max(an, b->chains[ci].n);
-        aN = std::max(aN, b->chains[ci].N);
-        amin = std::min(amin, b->chains[ci].n);
+    mg.d_desc = const_cast<ChainDesc*>(d_desc) + set_off[set_of[t]];
+    mg.chains.assign(na, 0);
+    mg.n = an;
+    mg.N = aN;
+    // one workgroup per CU over all active chains
+    const int per = std::max(1, 256 / na);
+    mg.wg_qr = std::max(1, std::min({per, amin / mg.qr_nb, kMaxWG}));
+    mg.wg_local = std::max(1, std::min({per, amin / 4, kMaxWG}));
+    if (local_window_sites(an, mg.wg_local) < 2) mg.wg_local = std::max(1, std::min({per, amin / 8, kMaxWG}));
+    Ops o(mg);
+    const ChainDesc* dd = mg.d_desc;
+    const int* sel = d_sel + (size_t)t * kMergedApplies * nch;
+    const int* sli = d_sli + (size_t)t * kMergedApplies * nch;
+    auto apply = [&](MatRef X, BOp op0, BOp op1, int a) -> int {
+      ProfScope ps(mg, DQMC_K_APPLY_B);
+      LCHK(launch_apply_B_chkr_sel(mg.stream, dd, na, an, aN, X, op0, op1, sel + (size_t)a * nch, sli + (size_t)a * nch,
+                                   mg.ngroups));
+      return 0;
+    };
+    auto P = [&](int role) { return MatRef{MK_PLAN, t * kPlanRoles + role}; };
+    auto PV = [&](int role) { return VecRef{VK_PLAN, t * kPlanRoles + role}; };
+    if (kinds[t] == 0) {
+      RC(apply(NM(M_G), B_LEFT, B_INV_LEFT, 0));
+      RC(apply(NM(M_G), B_INV_RIGHT, B_RIGHT, 1));
+    } else {
+      RC(o.copy(P(PR_PRE_DU), P(PR_PRE_SU)));
+      RC(o.copy(P(PR_PRE_DT), P(PR_PRE_ST)));
+      RC(o.copyv(PV(PR_PRE_DD), PV(PR_PRE_SD)));
+      if (mg.all_checks) {
+        RC(o.copy(P(PR_GW), NM(M_G)));
+        RC(apply(NM(M_GW), B_LEFT, B_LEFT, 0));
+        RC(apply(NM(M_GW), B_INV_RIGHT, B_INV_RIGHT, 1));
       }
-      mg.d_desc = b->d_mdesc + set_off[set_of[t]];
-      mg.chains.assign(na, 0);
-      mg.n = an;
-      mg.N = aN;
-      // one workgroup per CU over all active chains
-      const int per = std::max(1, 256 / na);
-      mg.wg_qr = std::max(1, std::min({per, amin / mg.qr_nb, kMaxWG}));
-      mg.wg_local = std::max(1, std::min({per, amin / 4, kMaxWG}));
-      if (local_window_sites(an, mg.wg_local) < 2) mg.wg_local = std::max(1, std::min({per, amin / 8, kMaxWG}));
-      Ops o(mg);
-      const ChainDesc* dd = mg.d_desc;
-      const int* sel = d_sel + (size_t)t * kMergedApplies * nch;
-      const int* sli = d_sli + (size_t)t * kMergedApplies * nch;
-      auto apply = [&](MatRef X, BOp op0, BOp op1, int a) -> int {
-        ProfScope ps(mg, DQMC_K_APPLY_B);
-        LCHK(launch_apply_B_chkr_sel(mg.stream, dd, na, an, aN, X, op0, op1, sel + (size_t)a * nch,
-                                     sli + (size_t)a * nch, mg.ngroups));
-        return 0;
-      };
-      auto P = [&](int role) { return MatRef{MK_PLAN, t * kPlanRoles + role}; };
-      auto PV = [&](int role) { return VecRef{VK_PLAN, t * kPlanRoles + role}; };
-      if (kinds[t] == 0) {
-        RC(apply(NM(M_G), B_LEFT, B_INV_LEFT, 0));
-        RC(apply(NM(M_G), B_INV_RIGHT, B_RIGHT, 1));
-      } else {
-        RC(o.copy(P(PR_PRE_DU), P(PR_PRE_SU)));
-        RC(o.copy(P(PR_PRE_DT), P(PR_PRE_ST)));
-        RC(o.copyv(PV(PR_PRE_DD), PV(PR_PRE_SD)));
-        if (mg.all_checks) {
-          RC(o.copy(P(PR_GW), NM(M_G)));
-          RC(apply(NM(M_GW), B_LEFT, B_LEFT, 0));
-          RC(apply(NM(M_GW), B_INV_RIGHT, B_INV_RIGHT, 1));
-        }
-        RC(o.copy(NM(M_CURU), P(PR_ADD_SU)));
-        for (int j = 0; j < s; ++j) RC(apply(NM(M_CURU), B_LEFT, B_DAG_LEFT, 2 + j));
-        LCHK(launch_scale_cols(mg.stream, dd, na, an, NM(M_CURU), PV(PR_ADD_SD)));
-        RC(o.udt(NM(M_CURU), PV(PR_ADD_DD), P(PR_ADD_DU), NM(M_QRT), true));
-        RC(o.gemm(P(PR_ADD_DT), NM(M_QRT), OP_N, P(PR_ADD_ST), OP_N));
-        RC(o.copy(P(PR_P1_DU), P(PR_P1_SU)));
-        RC(o.copy(P(PR_P1_DT), P(PR_P1_ST)));
-        RC(o.copyv(PV(PR_P1_DD), PV(PR_P1_SD)));
-        LCHK(launch_identity(mg.stream, dd, na, an, P(PR_ID_U), NOVEC));
-        LCHK(launch_identity(mg.stream, dd, na, an, P(PR_ID_T), NOVEC));
-        LCHK(launch_ones_vec(mg.stream, dd, na, an, PV(PR_ID_D)));
-        RC(o.copy(P(PR_P2_DU), P(PR_P2_SU)));
-        RC(o.copy(P(PR_P2_DT), P(PR_P2_ST)));
-        RC(o.copyv(PV(PR_P2_DD), PV(PR_P2_SD)));
-        RC(o.calculate_greens(true));
-        if (mg.all_checks) LCHK(launch_maxdiff(mg.stream, dd, na, an, P(PR_GW), NM(M_G)));
-        RC(apply(NM(M_G), B_INV_LEFT, B_INV_LEFT, 2 + s));
-        RC(apply(NM(M_G), B_RIGHT, B_RIGHT, 3 + s));
-      }
-      {
-        ProfScope ps(mg, DQMC_K_LOCAL);
-        int variant = mg.local_variant;
-        if (variant == 0)
-          variant = local_window_sites(an, mg.wg_local) >= 2 ? 4 : local_tile_fits(an, mg.wg_local) ? 2
-                    : local_delayed_fits(an) ? 3 : 1;
-        LCHK(launch_local_updates(mg.stream, dd, na, an, 0, d_loc + (size_t)t * nch, box, mg.wg_local, dolog,
-                                  variant));
-      }
+      RC(o.copy(NM(M_CURU), P(PR_ADD_SU)));
+      for (int j = 0; j < s; ++j) RC(apply(NM(M_CURU), B_LEFT, B_DAG_LEFT, 2 + j));
+      LCHK(launch_scale_cols(mg.stream, dd, na, an, NM(M_CURU), PV(PR_ADD_SD)));
+      RC(o.udt(NM(M_CURU), PV(PR_ADD_DD), P(PR_ADD_DU), NM(M_QRT), true));
+      RC(o.gemm(P(PR_ADD_DT), NM(M_QRT), OP_N, P(PR_ADD_ST), OP_N));
+      RC(o.copy(P(PR_P1_DU), P(PR_P1_SU)));
+      RC(o.copy(P(PR_P1_DT), P(PR_P1_ST)));
+      RC(o.copyv(PV(PR_P1_DD), PV(PR_P1_SD)));
+      LCHK(launch_identity(mg.stream, dd, na, an, P(PR_ID_U), NOVEC));
+      LCHK(launch_identity(mg.stream, dd, na, an, P(PR_ID_T), NOVEC));
+      LCHK(launch_ones_vec(mg.stream, dd, na, an, PV(PR_ID_D)));
+      RC(o.copy(P(PR_P2_DU), P(PR_P2_SU)));
+      RC(o.copy(P(PR_P2_DT), P(PR_P2_ST)));
+      RC(o.copyv(PV(PR_P2_DD), PV(PR_P2_SD)));
+      RC(o.calculate_greens(true));
+      if (mg.all_checks) LCHK(launch_maxdiff(mg.stream, dd, na, an, P(PR_GW), NM(M_G)));
+      RC(apply(NM(M_G), B_INV_LEFT, B_INV_LEFT, 2 + s));
+      RC(apply(NM(M_G), B_RIGHT, B_RIGHT, 3 + s));
+    }
+    {
+      ProfScope ps(mg, DQMC_K_LOCAL);
+      int variant = mg.local_variant;
+      if (variant == 0)
+        variant = local_window_sites(an, mg.wg_local) >= 2 ? 4 : local_tile_fits(an, mg.wg_local) ? 2
+                  : local_delayed_fits(an) ? 3 : 1;
+      LCHK(launch_local_updates(mg.stream, dd, na, an, 0, d_loc + (size_t)t * nch, box, mg.wg_local, dolog, variant));
     }
   }
-  // the groups' later work waits for the merged stream
+  // timing records of the pseudo group belong to the first group (dqmc_batch_profile_read)
+  if (!mg.prof.empty()) {
+    Group& gp0 = b->groups[0];
+    gp0.prof.insert(gp0.prof.end(), mg.prof.begin(), mg.prof.end());
+    mg.prof.clear();
+  }
+  return 0;
+}
+
+// the merged stream waits for the groups' queued work, and later group work for it
+int merged_enter(dqmc_batch* b) {
+  for (auto& g : b->groups) {
+    hipEvent_t e = g.take_event();
+    HIPCHK(hipEventRecord(e, g.stream));
+    HIPCHK(hipStreamWaitEvent(b->mstream, e, 0));
+    g.event_pool.push_back(e);
+  }
+  return 0;
+}
+int merged_leave(dqmc_batch* b) {
   hipEvent_t e = nullptr;
   HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipEventRecord(e, b->mstream));
   for (auto& g : b->groups) HIPCHK(hipStreamWaitEvent(g.stream, e, 0));
   (void)hipEventDestroy(e);
-  (void)nmin;
   return 0;
+}
+
+// Enqueue nsweeps sweeps of every chain of the batch (all groups compatible) on b->mstream,
+// planned in chunks of Mmax steps through one reusable plan buffer.
+int merged_sweeps(dqmc_batch* b, int nsweeps, double box, int dolog) {
+  const MergedShape ms = merged_shape(b);
+  if (ms.s > 16) return fail(DQMC_E_ARG, "merged sweeps need safe_mult <= 16");
+  const long long total = (long long)nsweeps * ms.Mmax;
+  if (total == 0) return 0;
+  const int TC = (int)std::min<long long>(total, ms.Mmax);
+  const size_t need = ms.ints(TC);
+  HIPCHK(hipStreamSynchronize(b->mstream));  // the previous call's buffers and host rows are free
+  if (need > b->mplan_cap) {
+    if (b->d_mplan) HIPCHK(hipFree(b->d_mplan));
+    b->d_mplan = nullptr;
+    HIPCHK(hipMalloc(&b->d_mplan, need * sizeof(int)));
+    b->mplan_cap = need;
+  }
+  size_t ndesc = 0;
+  for (auto& v : ms.sets) ndesc += v.size();
+  if (ndesc > b->mdesc_cap) {
+    if (b->d_mdesc) HIPCHK(hipFree(b->d_mdesc));
+    b->d_mdesc = nullptr;
+    HIPCHK(hipMalloc(&b->d_mdesc, ndesc * sizeof(ChainDesc)));
+    b->mdesc_cap = ndesc;
+  }
+  std::vector<size_t> set_off;
+  merged_descs(b, ms, TC, b->d_mplan, b->h_mdesc, set_off);
+  RC(merged_enter(b));
+  HIPCHK(hipMemcpyAsync(b->d_mdesc, b->h_mdesc.data(), b->h_mdesc.size() * sizeof(ChainDesc), hipMemcpyHostToDevice,
+                        b->mstream));
+  std::vector<int> kinds, set_of;
+  for (long long t0 = 0; t0 < total; t0 += TC) {
+    const int T = (int)std::min<long long>(TC, total - t0);
+    // the host rows are copied asynchronously: the previous chunk's copy must be done
+    HIPCHK(hipStreamSynchronize(b->mstream));
+    RC(merged_plan(b, ms, nsweeps, t0, TC, b->h_rows, kinds, set_of));
+    HIPCHK(hipMemcpyAsync(b->d_mplan, b->h_rows.data(), need * sizeof(int), hipMemcpyHostToDevice, b->mstream));
+    RC(merged_enqueue(b, ms, T, b->d_mplan, b->d_mdesc, set_off, kinds, set_of, box, dolog));
+  }
+  return merged_leave(b);
+}
+
+// One merged sweep (every chain one sweep) replayed from a HIP graph. The launches are fixed
+// by the groups' (current_slice, direction) at the start, the box and the logging flag; each
+// captured sweep owns its plan and descriptor buffers.
+int merged_sweep_graph(dqmc_batch* b, double box, int dolog) {
+  std::vector<int> key;
+  for (auto& g : b->groups) {
+    key.push_back(g.cur);
+    key.push_back(g.dir);
+  }
+  // the graphs read copies of the descriptors: a changed descriptor (a new uniform buffer or
+  // count, measurement or log buffers) is copied into every graph's array before a replay
+  std::vector<ChainDesc> descs;
+  for (auto& cm : b->chains) descs.push_back(cm.d);
+  for (auto& e : b->mgraphs)
+    if (std::memcmp(e.descs.data(), descs.data(), descs.size() * sizeof(ChainDesc)) != 0) {
+      HIPCHK(hipStreamSynchronize(b->mstream));
+      const MergedShape ms = merged_shape(b);
+      std::vector<ChainDesc> hd;
+      std::vector<size_t> set_off;
+      merged_descs(b, ms, ms.Mmax, e.d_plan, hd, set_off);
+      HIPCHK(hipMemcpy(e.d_desc, hd.data(), hd.size() * sizeof(ChainDesc), hipMemcpyHostToDevice));
+      e.descs = descs;
+    }
+  for (auto& e : b->mgraphs)
+    if (e.key == key && e.box == box && e.dolog == dolog) {
+      RC(merged_enter(b));
+      HIPCHK(hipGraphLaunch(e.exec, b->mstream));
+      for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        b->groups[gi].cur = e.end[2 * gi];
+        b->groups[gi].dir = e.end[2 * gi + 1];
+      }
+      return merged_leave(b);
+    }
+  const MergedShape ms = merged_shape(b);
+  if (ms.s > 16) return fail(DQMC_E_ARG, "merged sweeps need safe_mult <= 16");
+  const int T = ms.Mmax;
+  std::vector<int> h, kinds, set_of;
+  std::vector<ChainDesc> hd;
+  std::vector<size_t> set_off;
+  const std::vector<int> start = key;
+  const int rc0 = merged_plan(b, ms, 1, 0, T, h, kinds, set_of);
+  auto restore = [&]() {
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+      b->groups[gi].cur = start[2 * gi];
+      b->groups[gi].dir = start[2 * gi + 1];
+    }
+  };
+  if (rc0) {
+    restore();
+    return rc0;
+  }
+  dqmc_batch::MergedGraph e;
+  e.key = key;
+  e.descs = descs;
+  e.box = box;
+  e.dolog = dolog;
+  for (auto& g : b->groups) {
+    e.end.push_back(g.cur);
+    e.end.push_back(g.dir);
+  }
+  size_t ndesc = 0;
+  for (auto& v : ms.sets) ndesc += v.size();
+  hipError_t me = hipMalloc(&e.d_plan, h.size() * sizeof(int));
+  if (me == hipSuccess) me = hipMalloc(&e.d_desc, ndesc * sizeof(ChainDesc));
+  if (me == hipSuccess) me = hipMemcpy(e.d_plan, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (me == hipSuccess) {
+    merged_descs(b, ms, T, e.d_plan, hd, set_off);
+    me = hipMemcpy(e.d_desc, hd.data(), hd.size() * sizeof(ChainDesc), hipMemcpyHostToDevice);
+  }
+  if (me != hipSuccess) {
+    if (e.d_plan) (void)hipFree(e.d_plan);
+    if (e.d_desc) (void)hipFree(e.d_desc);
+    restore();
+    return fail(DQMC_E_HIP, std::string("merged sweep graph buffers: ") + hipGetErrorString(me));
+  }
+  RC(merged_enter(b));
+  HIPCHK(hipStreamSynchronize(b->mstream));
+  HIPCHK(hipStreamBeginCapture(b->mstream, hipStreamCaptureModeRelaxed));
+  const int rc = merged_enqueue(b, ms, T, e.d_plan, e.d_desc, set_off, kinds, set_of, box, dolog);
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(b->mstream, &graph);
+  hipError_t ie = hipSuccess;
+  if (!rc && ce == hipSuccess) ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc || ce != hipSuccess || ie != hipSuccess) {
+    (void)hipFree(e.d_plan);
+    (void)hipFree(e.d_desc);
+    restore();
+    if (rc) return rc;
+    return fail(DQMC_E_HIP, std::string("merged sweep graph: ") + hipGetErrorString(ce != hipSuccess ? ce : ie));
+  }
+  b->mgraphs.push_back(e);
+  HIPCHK(hipGraphLaunch(e.exec, b->mstream));
+  return merged_leave(b);
 }
 
 // device addresses of a chain's matrices and vectors, as matptr/vecptr compute them on the device
@@ -1590,6 +1734,7 @@ int dqmc_batch_set_groups(dqmc_batch* b, int wg_qr, int wg_inv, int wg_local) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
   if (wg_qr > kMaxWG || wg_inv > kMaxWG || wg_local > kMaxWG) return fail(DQMC_E_ARG, "too many workgroups");
   RC(batch_sync(b));
+  b->drop_mgraphs();
   for (auto& g : b->groups) {
     g.drop_graphs();
     if (wg_qr > 0) g.wg_qr = wg_qr;
@@ -1608,6 +1753,7 @@ int dqmc_set_greens_route(dqmc_ctx* ctx, int route) {
   CTX_CHECK(ctx);
   if (route != 0 && route != 1) return fail(DQMC_E_ARG, "route must be 0 or 1");
   RC(batch_sync(ctx->b));
+  ctx->b->drop_mgraphs();
   ctx->b->groups[0].drop_graphs();
   ctx->b->groups[0].greens_route = route;
   return 0;
@@ -1617,6 +1763,7 @@ int dqmc_set_udt_route(dqmc_ctx* ctx, int route) {
   CTX_CHECK(ctx);
   if (route != 0 && route != 1) return fail(DQMC_E_ARG, "udt route must be 0 or 1");
   RC(batch_sync(ctx->b));
+  ctx->b->drop_mgraphs();
   for (auto& g : ctx->b->groups) {
     g.drop_graphs();
     g.udt_route = route;
@@ -1628,6 +1775,7 @@ int dqmc_batch_set_local_variant(dqmc_batch* b, int variant) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
   if (variant < 0 || variant > 4) return fail(DQMC_E_ARG, "local variant must be 0, 1, 2, 3 or 4");
   RC(batch_sync(b));
+  b->drop_mgraphs();
   for (auto& g : b->groups) {
     g.drop_graphs();
     g.local_variant = variant;
@@ -1641,6 +1789,7 @@ int dqmc_batch_set_qr_panel(dqmc_batch* b, int nb) {
   for (auto& g : b->groups)
     if (nb == 16 && g.n > 512) return fail(DQMC_E_ARG, "16-column QR panels need n <= 512");
   RC(batch_sync(b));
+  b->drop_mgraphs();
   for (auto& g : b->groups) {
     g.drop_graphs();
     g.qr_nb = nb ? nb : 8;  // the default of default_groups
@@ -2430,6 +2579,12 @@ int dqmc_batch_sweep(dqmc_batch* b, int nsweeps, const double* box) {
     if (ok) {
       int dolog = 0;
       for (auto& cm : b->chains) dolog |= cm.log_cap > 0;
+      bool graphs = b->use_graph;
+      for (auto& g : b->groups) graphs = graphs && g.prof_mask == 0;  // timing events stay eager
+      if (graphs) {
+        for (int sw = 0; sw < nsweeps; ++sw) RC(merged_sweep_graph(b, box[g0.chains[0]], dolog));
+        return 0;
+      }
       return merged_sweeps(b, nsweeps, box[g0.chains[0]], dolog);
     }
   }
@@ -2701,6 +2856,7 @@ int autotune_group(dqmc_batch* b, Group& g, double box, int reps, double* rep) {
   (void)hipEventDestroy(e1);
   if (rc) return rc;
   g.drop_graphs();
+  b->drop_mgraphs();
   if (best_local < 1e29f) {
     g.wg_local = best_w;
     g.local_variant = best_v;
@@ -2742,8 +2898,10 @@ int dqmc_batch_use_graph(dqmc_batch* b, int enable) {
   if (!b) return fail(DQMC_E_ARG, "null batch");
   RC(batch_sync(b));
   b->use_graph = enable != 0;
-  if (!b->use_graph)
+  if (!b->use_graph) {
     for (auto& g : b->groups) g.drop_graphs();
+    b->drop_mgraphs();
+  }
   return 0;
 }
 

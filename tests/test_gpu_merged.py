@@ -64,6 +64,36 @@ def test_merged_equals_groupwise(amd):
     bg.close()
 
 
+def test_merged_graph_replay_equals_eager(amd):
+    """Merged sweeps replayed from HIP graphs (one graph per start state of the groups, each
+    one sweep of every chain) against eager merged calls (each chain's sweeps back to back):
+    every chain runs the same steps, but the set of chains sharing a launch differs, and with
+    it the local-update windows, so G agrees to rounding (1e-12) while the fields, counters and
+    stream positions are identical. The second call's uniform count changes the descriptors,
+    which the cached graphs must pick up."""
+    mod = amd
+    ps = [config_params(mod, 4, 20), config_params(mod, 6, 40, lam=1.5), config_params(mod, 8, 20, r=0.0)]
+    _say("merged graph replay vs eager merged: 3 shapes, 4 sweeps")
+    bm, _ = _batch(mod, ps, True, 3500)
+    bg, _ = _batch(mod, ps, True, 3500)
+    bg.use_graph(True)
+    for b in (bm, bg):
+        b.sweep(1)
+        b.sweep(3, sync=False)
+        b.synchronize()
+    for k, p in enumerate(ps):
+        sm, sg = bm.stats(k), bg.stats(k)
+        assert sg.device_error == 0
+        assert (sm.current_slice, sm.direction) == (sg.current_slice, sg.direction), k
+        assert sm.accepted == sg.accepted and sg.proposed == 4 * p.N * p.slices, k
+        assert bm.streams[k].consumed == bg.streams[k].consumed, k
+        assert np.array_equal(bm.get_field(k), bg.get_field(k)), k
+        assert np.abs(bm.greens(k) - bg.greens(k)).max() < 1e-12, k
+        assert sg.drift_max < 1e-8 and abs(sm.drift_max - sg.drift_max) < 1e-12, k
+    bm.close()
+    bg.close()
+
+
 def test_merged_chain_matches_oracle(amd):
     """A chain of a merged batch (L=6, beta=2, with L=4, beta=4 and L=8, beta=3 beside it)
     reproduces the oracle's field, G and stream consumption after 3 sweeps."""
