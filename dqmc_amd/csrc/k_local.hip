@@ -912,7 +912,7 @@ __global__ __launch_bounds__(256) void k_local_updates_delayed(const ChainDesc* 
 // Same arithmetic as update_greens! (local_updates.jl:61-95) up to the order of sums.
 // ---------------------------------------------------------------------------
 constexpr int kWinMax = 12;     // window sites with LDS-resident own columns
-constexpr int kWinMaxGlob = 6;  // window sites otherwise (the end-of-window rows keep 4K in registers)
+constexpr int kWinMaxGlob = 6;  // window sites otherwise (Bo and H of the own columns bound it)
 constexpr size_t kWindowLdsMax = 160 * 1024 - 1024;  // the CU's LDS less this kernel's static part
 
 __host__ __device__ inline int window_bo_stride(int n, int wg) { return (n + wg - 1) / wg + 1; }
@@ -922,43 +922,79 @@ __host__ __device__ inline size_t window_lds_bytes(int n, int N, int wg, int K, 
   return slice_lds_doubles(N) * 8 + (t + (size_t)8 * K * ncs + (size_t)48 * K * K + (size_t)32 * K) * 16;
 }
 
-// End-of-window rows without LDS-resident columns: Gn[:, own] = Gc[:, own] + Yt W, with the
-// row of Yt (4 MB entries, zero past 4m) in registers and 4 own columns per step; m >= 1.
-template <int MB>
-__device__ inline void window_rows_glob(const gcplx* __restrict__ Gc, gcplx* __restrict__ Gn, const cplx* W,
-                                        const int* accs, int m, int n, int N, int cb0, int ncol, int ncs) {
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    cplx y[4 * MB];
+// End of a window without LDS-resident columns, on the FP64 matrix cores:
+// Gn[:, own] = Gc[:, own] + Yt W as transposed 16 x 16 tiles, D = W^T Yt^T + Gc^T, so that a
+// tile's lane (i, j) holds own column i, row j and every load and store of G covers 16
+// consecutive rows of a column. Each wave takes row blocks of 16; for its block the Yt^T
+// fragments of all accepted sites stay in registers (two doubles per site and lane) while it
+// walks the own columns two tiles at a time (four independent accumulator chains; the next
+// pair's G loads are issued before this pair's products). One complex k-step of four
+// (one accepted site's four columns of Yt) is four real MFMAs:
+// re += Wr Yr - Wi Yi, im += Wr Yi + Wi Yr. v_mfma_f64_16x16x4 lane layout: A(i, k) and
+// B(k, j) on lane i + 16 k (resp. j + 16 k), D(i, j) element v on lane j + 16 ((i - v * 4)):
+// i = lane / 16 + 4 v. m >= 1.
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ inline void window_end_mfma(const gcplx* __restrict__ Gc, gcplx* __restrict__ Gn, const cplx* W,
+                                       const int* accs, int m, int n, int N, int cb0, int ncol, int ncs) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nrb = (n + 15) >> 4, ncp = (ncol + 31) >> 5;  // row blocks, pairs of column tiles
+  for (int rb = wave; rb < nrb; rb += nw) {
+    const int row = rb * 16 + li, rr = min(row, n - 1);
+    double yr[kWinMaxGlob], yi[kWinMaxGlob];
 #pragma unroll
-    for (int t = 0; t < 4 * MB; ++t) {
-      const int b = min(t >> 2, m - 1), k = t & 3;
-      const int row = accs[b] + k * N;
-      cplx v = ld(Gc + (size_t)row * n + r);
-      if (r == row) v.re -= 1.0;
-      y[t] = (t < 4 * m) ? v : mk(0, 0);
+    for (int b = 0; b < kWinMaxGlob; ++b) {
+      const int col = accs[min(b, m - 1)] + lk * N;
+      cplx v = ld(Gc + (size_t)col * n + rr);
+      if (rr == col) v.re -= 1.0;
+      yr[b] = b < m ? v.re : 0.0;
+      yi[b] = b < m ? v.im : 0.0;
     }
-    // the next four columns' loads are issued before this group's products (one group of
-    // HBM latency in flight while computing)
-    cplx nxt[4];
+    // G tile loads of a pair of column tiles starting at own column c0
+    auto load_pair = [&](int c0, cplx (&g)[8]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) nxt[q] = ld(Gc + (size_t)(cb0 + min(q, ncol - 1)) * n + r);
-    for (int j0 = 0; j0 < ncol; j0 += 4) {
-      cplx acc[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = nxt[q];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) nxt[q] = ld(Gc + (size_t)(cb0 + min(j0 + 4 + q, ncol - 1)) * n + r);
-#pragma unroll
-      for (int t = 0; t < 4 * MB; ++t) {
-        const cplx* w = W + (size_t)min(t, 4 * m - 1) * ncs;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) cfma(acc[q], y[t], w[min(j0 + q, ncol - 1)]);
+      for (int v = 0; v < 8; ++v) {
+        const int col = min(c0 + lk + 4 * (v & 3) + 16 * (v >> 2), ncol - 1);
+        g[v] = ld(Gc + (size_t)(cb0 + col) * n + rr);
       }
+    };
+    cplx nxt[8];
+    load_pair(0, nxt);
+    for (int cp = 0; cp < ncp; ++cp) {
+      const int c0 = cp * 32;
+      v4d ar0, ai0, ar1, ai1;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (j0 + q < ncol) {
-          st_nt(Gn + (size_t)(cb0 + j0 + q) * n + r, acc[q]);
+      for (int v = 0; v < 4; ++v) {
+        ar0[v] = nxt[v].re;
+        ai0[v] = nxt[v].im;
+        ar1[v] = nxt[v + 4].re;
+        ai1[v] = nxt[v + 4].im;
+      }
+      if (cp + 1 < ncp) load_pair(c0 + 32, nxt);
+      const int w0 = min(c0 + li, ncol - 1), w1 = min(c0 + 16 + li, ncol - 1);
+#pragma unroll
+      for (int b = 0; b < kWinMaxGlob; ++b) {
+        if (b < m) {
+          const cplx* wr = W + (size_t)(4 * b + lk) * ncs;
+          const cplx x0 = wr[w0], x1 = wr[w1];
+          ar0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.re, yr[b], ar0, 0, 0, 0);
+          ai0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.re, yi[b], ai0, 0, 0, 0);
+          ar1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.re, yr[b], ar1, 0, 0, 0);
+          ai1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.re, yi[b], ai1, 0, 0, 0);
+          ar0 = __builtin_amdgcn_mfma_f64_16x16x4f64(-x0.im, yi[b], ar0, 0, 0, 0);
+          ai0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.im, yr[b], ai0, 0, 0, 0);
+          ar1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-x1.im, yi[b], ar1, 0, 0, 0);
+          ai1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.im, yr[b], ai1, 0, 0, 0);
         }
+      }
+      if (row < n) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int ca = c0 + lk + 4 * v, cb = ca + 16;
+          if (ca < ncol) st_nt(Gn + (size_t)(cb0 + ca) * n + row, mk(ar0[v], ai0[v]));
+          if (cb < ncol) st_nt(Gn + (size_t)(cb0 + cb) * n + row, mk(ar1[v], ai1[v]));
+        }
+      }
     }
   }
 }
@@ -1189,11 +1225,7 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
         st_nt(Gn + (size_t)cb0 * n + e, ld(Gc + (size_t)cb0 * n + e));
       }
     } else {
-      switch ((m + 1) >> 1) {
-        case 1: window_rows_glob<2>(Gc, Gn, Wb, accs, m, n, N, cb0, ncol, ncs); break;
-        case 2: window_rows_glob<4>(Gc, Gn, Wb, accs, m, n, N, cb0, ncol, ncs); break;
-        default: window_rows_glob<6>(Gc, Gn, Wb, accs, m, n, N, cb0, ncol, ncs); break;
-      }
+      window_end_mfma(Gc, Gn, Wb, accs, m, n, N, cb0, ncol, ncs);
     }
     if (rw + 1 < R) {
       group_sync(g);

@@ -409,7 +409,10 @@ def test_local_update_variants(amd, variant):
     mod, DQMC = amd
     cases = [(6, 3), (6, 8), (6, 36)]
     if variant == 4:
-        cases.append((8, 4))  # n = 256 at 4 workgroups: the window without LDS-resident columns
+        # the window without LDS-resident columns (end-of-window product on the matrix cores):
+        # n = 256 at 4 workgroups, and n = 324 (L = 9, generic checkerboard) at 2 workgroups,
+        # where neither the rows nor the 162 own columns fill whole 16 x 16 tiles
+        cases += [(8, 4), (9, 2)]
     for L, wg in cases:
         _say("local-update variant %d, %d workgroups, L=%d M=20, 3 sweeps" % (variant, wg, L))
         p = config_params(mod, L, 20)
