@@ -1077,10 +1077,26 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
     for (int e = tid; e < kw * K4 * 4; e += blockDim.x) Us[e] = mk(0, 0);
     for (int e = tid; e < kw * 4 * ncs; e += blockDim.x) Hs[e] = mk(0, 0);
     __syncthreads();
-    int m = 0;
+    // Per accepted proposal m at window site ii, two barriers: after the evaluation (S1) and
+    // after the factor pair (S2). The sums kept for the sites ahead are then brought up to date
+    // while wave 0 already evaluates the next proposals: wave 0 updates the diagonal blocks
+    // G'(r_j, c_j) it reads, waves 1-3 everything else (off-diagonal blocks, U, H); the next
+    // S1 completes both before the factor pair of the next acceptance reads them.
+    int m = 0, pend = -1;  // pend: window site of the acceptance whose sums are still to update
     while (!stop) {
       if (tid < 64) {
         const int lane = tid & 63, ea = (lane & 15) >> 2, eb = lane & 3;
+        if (pend >= 0) {  // G'(r_j, c_j) += A_m(r_j, :) B_m(:, c_j), j > pend
+          const int ka = kw - pend - 1;
+          for (int e = lane; e < ka * 16; e += 64) {
+            const int jj = pend + 1 + (e >> 4), a = (e >> 2) & 3, b = e & 3;
+            cplx acc = Gp[(jj * K + jj) * 16 + a * 4 + b];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cfma(acc, Ar[jj * 16 + a * 4 + q], Bw[jj * 16 + q * 4 + b]);
+            Gp[(jj * K + jj) * 16 + a * 4 + b] = acc;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the wave's writes before its reads
+        }
         int st;
         const int site = evaluate_proposals(
             c, s, p, sh, box, dolog, w == 0,
@@ -1089,8 +1105,37 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
           sh.site = site;
           sh.stop = st;
         }
+      } else if (pend >= 0) {
+        // the pending acceptance (index m - 1) for the sites ahead (j, k > pend):
+        //   G'(r_j, c_k) += A(r_j, :) B(:, c_k) (j != k),  U_k += P B(:, c_k),  H_j += A(r_j, :) B(:, own)
+        const int t3 = tid - 64, n3 = blockDim.x - 64, mp = m - 1;
+        const int ka = kw - pend - 1;
+        for (int e = t3; e < ka * ka * 16; e += n3) {
+          const int jk = e >> 4, jj = pend + 1 + jk / ka, kk = pend + 1 + jk % ka, a = (e >> 2) & 3, b = e & 3;
+          if (jj == kk) continue;
+          cplx acc = Gp[(jj * K + kk) * 16 + a * 4 + b];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cfma(acc, Ar[jj * 16 + a * 4 + q], Bw[kk * 16 + q * 4 + b]);
+          Gp[(jj * K + kk) * 16 + a * 4 + b] = acc;
+        }
+        const int nu = 16 * m;
+        for (int e = t3; e < ka * nu; e += n3) {
+          const int kk = pend + 1 + e / nu, r2 = e % nu, t = r2 >> 2, q = r2 & 3;
+          cplx acc = Us[((size_t)kk * K4 + t) * 4 + q];
+#pragma unroll
+          for (int q2 = 0; q2 < 4; ++q2) cfma(acc, Pc[((size_t)mp * K4 + t) * 4 + q2], Bw[kk * 16 + q2 * 4 + q]);
+          Us[((size_t)kk * K4 + t) * 4 + q] = acc;
+        }
+        const int nh = 4 * ncol;
+        for (int e = t3; e < ka * nh; e += n3) {
+          const int jj = pend + 1 + e / nh, r2 = e % nh, q = r2 / ncol, jl = r2 - q * ncol;
+          cplx acc = Hs[((size_t)jj * 4 + q) * ncs + jl];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) cfma(acc, Ar[jj * 16 + q * 4 + k], Bo[((size_t)mp * 4 + k) * ncs + jl]);
+          Hs[((size_t)jj * 4 + q) * ncs + jl] = acc;
+        }
       }
-      __syncthreads();
+      __syncthreads();  // S1
       const int site = sh.site;
       stop = sh.stop;
       if (site >= s1) break;
@@ -1138,35 +1183,9 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
         Pc[((size_t)m * K4 + t) * 4 + k] = acc;
       }
       if (tid == 0) accs[m] = site;
-      __syncthreads();
-      // the sums kept for the sites ahead (j, k > i):
-      //   G'(r_j, c_k) += A_m(r_j, :) B_m(:, c_k),  U_k += P_m B_m(:, c_k),  H_j += A_m(r_j, :) B_m(:, own)
-      const int ka = kw - ii - 1;
-      for (int e = tid; e < ka * ka * 16; e += blockDim.x) {
-        const int jk = e >> 4, jj = ii + 1 + jk / ka, kk = ii + 1 + jk % ka, a = (e >> 2) & 3, b = e & 3;
-        cplx acc = Gp[(jj * K + kk) * 16 + a * 4 + b];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) cfma(acc, Ar[jj * 16 + a * 4 + q], Bw[kk * 16 + q * 4 + b]);
-        Gp[(jj * K + kk) * 16 + a * 4 + b] = acc;
-      }
-      const int nu = 16 * (m + 1);
-      for (int e = tid; e < ka * nu; e += blockDim.x) {
-        const int kk = ii + 1 + e / nu, r2 = e % nu, t = r2 >> 2, q = r2 & 3;
-        cplx acc = Us[((size_t)kk * K4 + t) * 4 + q];
-#pragma unroll
-        for (int q2 = 0; q2 < 4; ++q2) cfma(acc, Pc[((size_t)m * K4 + t) * 4 + q2], Bw[kk * 16 + q2 * 4 + q]);
-        Us[((size_t)kk * K4 + t) * 4 + q] = acc;
-      }
-      const int nh = 4 * ncol;
-      for (int e = tid; e < ka * nh; e += blockDim.x) {
-        const int jj = ii + 1 + e / nh, r2 = e % nh, q = r2 / ncol, jl = r2 - q * ncol;
-        cplx acc = Hs[((size_t)jj * 4 + q) * ncs + jl];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cfma(acc, Ar[jj * 16 + q * 4 + k], Bo[((size_t)m * 4 + k) * ncs + jl]);
-        Hs[((size_t)jj * 4 + q) * ncs + jl] = acc;
-      }
+      __syncthreads();  // S2
+      pend = ii;
       ++m;
-      __syncthreads();
     }
     // W = sum_b P_b B_b (into the H storage, no longer needed): entry (t, jl) sums over the b
     // with P_b nonzero in row t (b >= t / 4)
