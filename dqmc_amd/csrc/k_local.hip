@@ -253,7 +253,35 @@ struct Proposals {
   long long cur, logi;
   double dS_acc, minratio;
   int nacc, nonreal, i;
+  // an accepted proposal whose field update is deferred (commit_accept): site (-1: none),
+  // exp(-dS) and the new field
+  int pend_site;
+  double pend_e, pend_n0, pend_n1, pend_n2;
 };
+
+// The accepted proposal's field update (wave 0): the field of the site, the four neighbours'
+// space sums recomputed in the original order (action.jl:57-101), and the boson action
+// p.boson_action += -log(exp_delta_S_boson) (local_updates.jl:34). The next evaluation reads
+// the field and sums, nothing before it does.
+__device__ inline void commit_accept(const ChainDesc& c, const SliceLds& s, Proposals& p) {
+  const int site = p.pend_site;
+  if (site < 0) return;
+  const int N = c.N, lane = threadIdx.x & 63;
+  if (lane == 0) {
+    s.phl[3 * site + 0] = p.pend_n0;
+    s.phl[3 * site + 1] = p.pend_n1;
+    s.phl[3 * site + 2] = p.pend_n2;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // lane 0's field before the sums read it
+  if (lane < 12) {
+    const int j = s.nbl[(lane / 3) * N + site], a = lane - 3 * (lane / 3);
+    double sn = 0.0;
+    for (int k = 0; k < 4; ++k) sn += s.phl[3 * s.nbl[k * N + j] + a];
+    s.snb[3 * j + a] = sn;
+  }
+  p.dS_acc += -log(p.pend_e);
+  p.pend_site = -1;
+}
 
 // Evaluate proposals from site p.i on, in wave 0 (each 16-lane row of the wave holds one
 // site's 4x4 algebra, lane (ea, eb) of the row one entry), until one is accepted: its delta
@@ -266,7 +294,8 @@ struct Proposals {
 // calc_boson_action_diff (action.jl:57-101).
 template <class Diag>
 __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposals& p, LocalShared& sh,
-                                  double box, int dolog, bool logger, Diag diag, int* stop, int limit) {
+                                  double box, int dolog, bool logger, Diag diag, int* stop, int limit,
+                                  bool defer = false) {
   const int N = c.N, lane = threadIdx.x & 63;
   const int row = lane >> 4;  // 16-lane row of the wave: it evaluates site i + row
   const int ea = (lane & 15) >> 2, eb = lane & 3;
@@ -393,20 +422,12 @@ __device__ int evaluate_proposals(const ChainDesc& c, const SliceLds& s, Proposa
       ++p.logi;
       if (acc) {
         ++p.nacc;
-        p.dS_acc += -log(read_lane(exp_dS, 16 * g));
-        const double n0 = read_lane(nw[0], 16 * g), n1 = read_lane(nw[1], 16 * g), n2 = read_lane(nw[2], 16 * g);
-        if (lane == 0) {
-          s.phl[3 * site + 0] = n0;
-          s.phl[3 * site + 1] = n1;
-          s.phl[3 * site + 2] = n2;
-        }
-        // the four neighbours' space sums, recomputed in the original order
-        if (lane < 12) {
-          const int j = s.nbl[(lane / 3) * N + site], a = lane - 3 * (lane / 3);
-          double sn = 0.0;
-          for (int k = 0; k < 4; ++k) sn += s.phl[3 * s.nbl[k * N + j] + a];
-          s.snb[3 * j + a] = sn;
-        }
+        p.pend_site = site;
+        p.pend_e = read_lane(exp_dS, 16 * g);
+        p.pend_n0 = read_lane(nw[0], 16 * g);
+        p.pend_n1 = read_lane(nw[1], 16 * g);
+        p.pend_n2 = read_lane(nw[2], 16 * g);
+        if (!defer) commit_accept(c, s, p);
         // M^-1 (b, a) = cofactor (a, b) / det M, one entry per lane of row g
         cplx m3[9];
 #pragma unroll
@@ -471,7 +492,7 @@ __global__ __launch_bounds__(256) void k_local_updates(const ChainDesc* chains, 
   __shared__ LocalShared sh;
   __syncthreads();
 
-  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0, -1, 0.0, 0.0, 0.0, 0.0};
   int buf = 0;
   cplx* Gb[2] = {c.mat[M_G], c.mat[M_GALT]};
   for (;;) {
@@ -574,7 +595,7 @@ __global__ __launch_bounds__(256) void k_local_updates_tile(const ChainDesc* cha
   __syncthreads();
 
   const size_t win_sz = (size_t)kLocalWindow * 4 * n;  // complex entries of one window
-  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0, -1, 0.0, 0.0, 0.0, 0.0};
   int rnd = 0;  // completed rounds; buffers of round r live at parity r & 1
   int stop = 0;
 
@@ -767,7 +788,7 @@ __global__ __launch_bounds__(256) void k_local_updates_delayed(const ChainDesc* 
     Gd[e] = G[(size_t)(j + b * N) * n + (j + a * N)];
   }
   __syncthreads();
-  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0, -1, 0.0, 0.0, 0.0, 0.0};
   int m = 0, stop = 0;
   cplx* As = c.ld_A;
   cplx* Bs = c.ld_B;
@@ -1043,7 +1064,7 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
     }
   }
 
-  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0};
+  Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0, -1, 0.0, 0.0, 0.0, 0.0};
   int stop = 0;  // the uniform stream ran out: later windows make no proposals
   for (int rw = 0; rw < R; ++rw) {
     const int s0 = (int)((long long)rw * N / R), s1 = (int)((long long)(rw + 1) * N / R), kw = s1 - s0;
@@ -1100,7 +1121,7 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
         int st;
         const int site = evaluate_proposals(
             c, s, p, sh, box, dolog, w == 0,
-            [&](int j) { return Gp[((j - s0) * (K + 1)) * 16 + ea * 4 + eb]; }, &st, s1);
+            [&](int j) { return Gp[((j - s0) * (K + 1)) * 16 + ea * 4 + eb]; }, &st, s1, true);
         if (tid == 0) {
           sh.site = site;
           sh.stop = st;
@@ -1140,49 +1161,55 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
       stop = sh.stop;
       if (site >= s1) break;
       const int ii = site - s0;
-      // B_m on own columns: delta (G0(r_i, .) + H_i)
-      for (int jl = tid; jl < ncol; jl += blockDim.x) {
-        cplx g4[4];
+      if (tid < 64) {
+        // wave 0: the accepted proposal's field update, while waves 1-3 form its factor pair
+        commit_accept(c, s, p);
+      } else {
+        const int t3 = tid - 64, n3 = blockDim.x - 64;
+        // B_m on own columns: delta (G0(r_i, .) + H_i)
+        for (int jl = t3; jl < ncol; jl += n3) {
+          cplx g4[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          g4[q] = TILE ? T[(size_t)jl * n + site + q * N] : ld(Gc + (size_t)(cb0 + jl) * n + site + q * N);
-          g4[q] += Hs[((size_t)ii * 4 + q) * ncs + jl];
+          for (int q = 0; q < 4; ++q) {
+            g4[q] = TILE ? T[(size_t)jl * n + site + q * N] : ld(Gc + (size_t)(cb0 + jl) * n + site + q * N);
+            g4[q] += Hs[((size_t)ii * 4 + q) * ncs + jl];
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            cplx acc = mk(0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cfma(acc, sh.delta[k * 4 + q], g4[q]);
+            Bo[((size_t)m * 4 + k) * ncs + jl] = acc;
+          }
         }
+        // B_m(:, c_k) = delta G'(r_i, c_k) and A_m(r_j, :) = G'(r_j, c_i) M^-1 for the sites
+        // ahead (j, k > i: the only ones read again)
+        for (int e = t3; e < kw * 16; e += n3) {
+          const int kk = e >> 4, k = (e >> 2) & 3, b = e & 3;
+          if (kk <= ii) continue;
+          cplx bw = mk(0, 0), ar = mk(0, 0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+          for (int q = 0; q < 4; ++q) {
+            cfma(bw, sh.delta[k * 4 + q], Gp[(ii * K + kk) * 16 + q * 4 + b]);
+            cfma(ar, Gp[(kk * K + ii) * 16 + k * 4 + q], sh.minv[q * 4 + b]);
+          }
+          Bw[kk * 16 + k * 4 + b] = bw;
+          Ar[kk * 16 + k * 4 + b] = ar;
+        }
+        // P_m = (S_m + U_i) M^-1 (rows of blocks <= m)
+        for (int e = t3; e < 16 * (m + 1); e += n3) {
+          const int t = e >> 2, k = e & 3;
           cplx acc = mk(0, 0);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) cfma(acc, sh.delta[k * 4 + q], g4[q]);
-          Bo[((size_t)m * 4 + k) * ncs + jl] = acc;
+          for (int q = 0; q < 4; ++q) {
+            cplx v = Us[((size_t)ii * K4 + t) * 4 + q];
+            if (t == 4 * m + q) v.re += 1.0;
+            cfma(acc, v, sh.minv[q * 4 + k]);
+          }
+          Pc[((size_t)m * K4 + t) * 4 + k] = acc;
         }
+        if (t3 == 0) accs[m] = site;
       }
-      // B_m(:, c_k) = delta G'(r_i, c_k) and A_m(r_j, :) = G'(r_j, c_i) M^-1 for the sites
-      // ahead (j, k > i: the only ones read again)
-      for (int e = tid; e < kw * 16; e += blockDim.x) {
-        const int kk = e >> 4, k = (e >> 2) & 3, b = e & 3;
-        if (kk <= ii) continue;
-        cplx bw = mk(0, 0), ar = mk(0, 0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          cfma(bw, sh.delta[k * 4 + q], Gp[(ii * K + kk) * 16 + q * 4 + b]);
-          cfma(ar, Gp[(kk * K + ii) * 16 + k * 4 + q], sh.minv[q * 4 + b]);
-        }
-        Bw[kk * 16 + k * 4 + b] = bw;
-        Ar[kk * 16 + k * 4 + b] = ar;
-      }
-      // P_m = (S_m + U_i) M^-1 (rows of blocks <= m)
-      for (int e = tid; e < 16 * (m + 1); e += blockDim.x) {
-        const int t = e >> 2, k = e & 3;
-        cplx acc = mk(0, 0);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          cplx v = Us[((size_t)ii * K4 + t) * 4 + q];
-          if (t == 4 * m + q) v.re += 1.0;
-          cfma(acc, v, sh.minv[q * 4 + k]);
-        }
-        Pc[((size_t)m * K4 + t) * 4 + k] = acc;
-      }
-      if (tid == 0) accs[m] = site;
       __syncthreads();  // S2
       pend = ii;
       ++m;
