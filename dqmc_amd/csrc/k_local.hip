@@ -1037,7 +1037,8 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
   const SliceLds s = load_slice(c, l, smem);
   cplx* T = reinterpret_cast<cplx*>(smem + slice_lds_doubles(N));  // [ncs - 1][n] own columns (TILE)
   cplx* Bo = T + (TILE ? (size_t)(ncs - 1) * n : 0);  // [K][4][ncs] B_b on own columns
-  cplx* Hs = Bo + (size_t)K4 * ncs;  // [K][4][ncs] H_j = sum_b A_b(r_j, :) B_b(:, own); W at a window's end
+  // [K][4][ncs] H_j = G'(r_j, own) = G0(r_j, own) + sum_b A_b(r_j, :) B_b(:, own); W at a window's end
+  cplx* Hs = Bo + (size_t)K4 * ncs;
   cplx* Gp = Hs + (size_t)K4 * ncs;  // [K][K][16]  G'(r_j, c_k)
   cplx* Pc = Gp + (size_t)16 * K * K;  // [K][4K][4] P_b
   cplx* Us = Pc + (size_t)16 * K * K;  // [K][4K][4] U_k = sum_b P_b B_b(:, c_k)
@@ -1062,6 +1063,7 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
           T[e0 + q * blockDim.x].im = v[q].im;  // are lowered through a stack slot)
         }
     }
+    __syncthreads();  // the first window's H rows are read from T
   }
 
   Proposals p{s.cur0, dolog ? *c.log_count : 0, 0.0, c.scal[S_MINRATIO], 0, 0, 0, -1, 0.0, 0.0, 0.0, 0.0};
@@ -1096,7 +1098,25 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
       }
     }
     for (int e = tid; e < kw * K4 * 4; e += blockDim.x) Us[e] = mk(0, 0);
-    for (int e = tid; e < kw * 4 * ncs; e += blockDim.x) Hs[e] = mk(0, 0);
+    // H_j starts as the window's row r_j of G0 on the own columns (so an acceptance reads its
+    // site's rows from LDS); four loads per thread in flight
+    for (int e0 = tid; e0 < kw * 4 * ncs; e0 += 4 * blockDim.x) {
+      cplx v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = min(e0 + u * (int)blockDim.x, kw * 4 * ncs - 1);
+        const int jq = e / ncs, jl = min(e - jq * ncs, ncol - 1), row = s0 + (jq >> 2) + (jq & 3) * N;
+        v[u] = TILE ? T[(size_t)jl * n + row] : ld(Gc + (size_t)(cb0 + jl) * n + row);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + u * (int)blockDim.x;
+        if (e < kw * 4 * ncs) {
+          Hs[e].re = v[u].re;
+          Hs[e].im = v[u].im;
+        }
+      }
+    }
     __syncthreads();
     // Per accepted proposal m at window site ii, two barriers: after the evaluation (S1) and
     // after the factor pair (S2). The sums kept for the sites ahead are then brought up to date
@@ -1166,14 +1186,11 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
         commit_accept(c, s, p);
       } else {
         const int t3 = tid - 64, n3 = blockDim.x - 64;
-        // B_m on own columns: delta (G0(r_i, .) + H_i)
+        // B_m on own columns: delta G'(r_i, own) = delta H_i
         for (int jl = t3; jl < ncol; jl += n3) {
           cplx g4[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            g4[q] = TILE ? T[(size_t)jl * n + site + q * N] : ld(Gc + (size_t)(cb0 + jl) * n + site + q * N);
-            g4[q] += Hs[((size_t)ii * 4 + q) * ncs + jl];
-          }
+          for (int q = 0; q < 4; ++q) g4[q] = Hs[((size_t)ii * 4 + q) * ncs + jl];
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
             cplx acc = mk(0, 0);
