@@ -36,7 +36,8 @@ __device__ inline cplx read_lane_c(cplx v, int lane) { return mk(read_lane(v.re,
 // zeros above the diagonal), columns t >= kb zero-filled.
 template <int NB>
 __device__ inline void stage_panel(cplx* P, const gcplx* A, const int* perm, int k0, int kb, int n, int tid,
-                                   int mode) {
+                                   int mode, int cs = 0) {
+  if (cs == 0) cs = n;  // LDS column stride of P
   const gcplx* src[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
@@ -51,7 +52,7 @@ __device__ inline void stage_panel(cplx* P, const gcplx* A, const int* perm, int
     for (int t = 0; t < NB; ++t) {
       cplx val = v[t];
       if (mode == 1) val = (t >= kb || r < k0 + t) ? mk(0, 0) : (r == k0 + t) ? mk(1, 0) : val;
-      P[(size_t)t * n + r] = val;
+      P[(size_t)t * cs + r] = val;
     }
   }
 }
@@ -642,6 +643,271 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Panel application on the FP64 matrix cores for LDS-resident column blocks (k_qr_solve_mfma,
+// k_form_Q_mfma). The workgroup owns CB <= 16 columns of X (one 16-column MFMA tile) and
+// applies one 8-column compact-WY panel, X <- (I - V op(T) V^H) X on the rows >= k0:
+//   W = V^H X (8 x CB): v_mfma_f64_16x16x4 with A(i, k) = conj V(r, i) (rows 8..15 of the
+//     tile zero) and B(k, j) = X(r, j), the rows split into four contiguous ranges, one per
+//     wave; waves 1-3 leave their partial W in LDS, wave 0 adds them, forms Z = op(T) W with
+//     the vector units and leaves Z in LDS;
+//   X -= V Z: 16-row tiles of X spread over the waves, A(i, k) = V(r0 + i, k), B(k, j) =
+//     Z(k, j), two k-steps.
+// A complex product is four real MFMAs. Lane layout of v_mfma_f64_16x16x4: A(i, k) and
+// B(k, j) on lane i + 16 k (resp. j + 16 k); D(i, j) element v on lane j + 16 (i - 4 v).
+// LDS: X [CB][ld], V [8][ld] (unit lower trapezoid from stage_panel), 3 x 8 x 16 scratch.
+// ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kMfmaScratch = 3 * 8 * 16;  // complex entries of the panel scratch
+
+__device__ inline v4d mfma4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+template <int CB, bool ADJ>
+__device__ inline void apply_panel_mfma(cplx* xc, const cplx* V, const cplx* Tl, cplx* scr, int n, int cs,
+                                        int k0) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int vi = min(li, 7), xj = min(li, CB - 1);
+  const int nst = (n - k0) >> 2;  // n - k0 is a multiple of 4
+  const int sa = (nst * wave) >> 2, sb = (nst * (wave + 1)) >> 2;
+  v4d wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
+  for (int st = sa; st < sb; ++st) {
+    const int r = k0 + 4 * st + lk;
+    const cplx v = V[(size_t)vi * cs + r];
+    const cplx x = xc[(size_t)xj * cs + r];
+    const double vr = li < 8 ? v.re : 0.0, vm = li < 8 ? v.im : 0.0;
+    const double xr = li < CB ? x.re : 0.0, xm = li < CB ? x.im : 0.0;
+    // W += V^H X: re += Vr Xr + Vi Xi, im += Vr Xi - Vi Xr
+    wr = mfma4(vr, xr, wr);
+    wi = mfma4(vr, xm, wi);
+    wr = mfma4(vm, xm, wr);
+    wi = mfma4(-vm, xr, wi);
+  }
+  // lane (li, lk) holds W(lk, li) (element 0) and W(lk + 4, li) (element 1)
+  if (wave > 0) {
+    cplx* q = scr + (wave - 1) * 128;
+    q[lk * 16 + li] = mk(wr[0], wi[0]);
+    q[(lk + 4) * 16 + li] = mk(wr[1], wi[1]);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    cplx w0 = mk(wr[0], wi[0]), w1 = mk(wr[1], wi[1]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      w0 += scr[q * 128 + lk * 16 + li];
+      w1 += scr[q * 128 + (lk + 4) * 16 + li];
+    }
+    // the full W over wave 1's partial (each lane rewrites only the entries it read)
+    scr[lk * 16 + li] = w0;
+    scr[(lk + 4) * 16 + li] = w1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    cplx z0 = mk(0, 0), z1 = mk(0, 0);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const cplx wl = scr[l * 16 + li];
+      if (ADJ) {  // Z = T^H W
+        cfma_cj(z0, Tl[l * 8 + lk], wl);
+        cfma_cj(z1, Tl[l * 8 + lk + 4], wl);
+      } else {  // Z = T W
+        cfma(z0, Tl[lk * 8 + l], wl);
+        cfma(z1, Tl[(lk + 4) * 8 + l], wl);
+      }
+    }
+    scr[128 + lk * 16 + li] = z0;
+    scr[128 + (lk + 4) * 16 + li] = z1;
+  }
+  __syncthreads();
+  const cplx* Zs = scr + 128;
+  const int nrb = (n - k0 + 15) >> 4;
+  for (int rb = wave; rb < nrb; rb += 4) {
+    const int r0 = k0 + 16 * rb;
+    v4d dr, di;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = min(r0 + lk + 4 * v, n - 1);
+      const cplx x = xc[(size_t)xj * cs + row];
+      dr[v] = x.re;
+      di[v] = x.im;
+    }
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int ra = r0 + li;
+      const cplx a0 = V[(size_t)(4 * ks + lk) * cs + min(ra, n - 1)];
+      const double ar = ra < n ? a0.re : 0.0, am = ra < n ? a0.im : 0.0;
+      const cplx z = Zs[(4 * ks + lk) * 16 + li];
+      // X -= V Z: re += -Vr Zr + Vi Zi, im += -Vr Zi - Vi Zr
+      dr = mfma4(-ar, z.re, dr);
+      di = mfma4(-ar, z.im, di);
+      dr = mfma4(am, z.im, dr);
+      di = mfma4(-am, z.re, di);
+    }
+    if (li < CB) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r0 + lk + 4 * v;
+        if (row < n) xc[(size_t)li * cs + row] = mk(dr[v], di[v]);
+      }
+    }
+  }
+}
+
+// k_qr_solve on the matrix cores: B <- R^-1 Q^H B with CB columns per workgroup. The panels'
+// Q_p^H as in apply_panel_mfma; the back substitution in blocks of 16 rows from the bottom:
+// each wave solves the 16 x 16 diagonal block for four columns at once (lane = row + 16 x
+// column; the block's inverse diagonal precomputed once), then the rows above take the
+// block's contribution as MFMA tiles, R(r, lo:hi) y(lo:hi, :) (K = 16).
+template <int CB>
+__global__ __launch_bounds__(256) void k_qr_solve_mfma(const ChainDesc* chains, MatRef Ar, MatRef Br, int cs) {
+  int bx, by;
+  xcd_block(bx, by);
+  const ChainDesc& c = chains[by];
+  const int n = c.n;
+  const int j0 = bx * CB;
+  if (j0 >= n) return;
+  const int ncol = min(CB, n - j0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const gcplx* A = gp(matptr(c, Ar));
+  gcplx* B = gp(matptr(c, Br));
+  extern __shared__ double smem[];
+  cplx* xc = reinterpret_cast<cplx*>(smem);  // [CB][cs]
+  cplx* V = xc + (size_t)CB * cs;             // [8][cs]; the back substitution's diagonal block
+  cplx* scr = V + (size_t)8 * cs;             // panel scratch
+  __shared__ cplx Tl[64];
+  for (int r = tid; r < n; r += 256) {
+    cplx v[CB];
+#pragma unroll
+    for (int q = 0; q < CB; ++q) v[q] = ld(B + (size_t)(j0 + min(q, ncol - 1)) * n + r);
+#pragma unroll
+    for (int q = 0; q < CB; ++q) {
+      xc[(size_t)q * cs + r].re = v[q].re;  // (a whole-struct copy from a register array is
+      xc[(size_t)q * cs + r].im = v[q].im;  // lowered through a stack slot)
+    }
+  }
+  for (int k0 = 0, p = 0; k0 < n; k0 += 8, ++p) {
+    const int kb = min(8, n - k0);
+    __syncthreads();
+    stage_panel<8>(V, A, nullptr, k0, kb, n, tid, 1, cs);  // columns >= kb zero-filled
+    if (tid < 64) Tl[tid] = ld(gp(c.qr_Tf) + (size_t)p * 64 + tid);
+    __syncthreads();
+    apply_panel_mfma<CB, true>(xc, V, Tl, scr, n, cs, k0);
+  }
+  cplx* Rd = V;        // [16][16]: Rd[kk * 16 + rr] = R(lo + rr, lo + kk)
+  cplx* Ri = V + 256;  // [16]: 1 / R(lo + k, lo + k)
+  const int xj = min(li, CB - 1);
+  for (int hi = n; hi > 0; hi -= 16) {
+    const int lo = max(0, hi - 16), bs = hi - lo;
+    __syncthreads();
+    {
+      const int kk = tid >> 4, rr = tid & 15;
+      Rd[tid] = (kk < bs && rr < bs) ? ld(A + (size_t)(lo + kk) * n + lo + rr) : mk(0, 0);
+      if (tid < 16) Ri[tid] = tid < bs ? cinv(ld(A + (size_t)(lo + tid) * n + lo + tid)) : mk(0, 0);
+    }
+    __syncthreads();
+    {  // wave w: columns w + 4 lk, lane row li
+      const int col = wave + 4 * lk;
+      cplx* y = xc + (size_t)min(col, CB - 1) * cs;
+      cplx zr = li < bs ? y[lo + li] : mk(0, 0);
+      for (int k = bs - 1; k >= 0; --k) {
+        const cplx yk = shfl_c(zr, (lane & 48) + k) * Ri[k];
+        if (li < k) zr = zr - Rd[k * 16 + li] * yk;
+        if (li == k) zr = yk;
+      }
+      if (col < CB && li < bs) y[lo + li] = zr;
+    }
+    __syncthreads();
+    const int nrb = (lo + 15) >> 4;
+    for (int rb = wave; rb < nrb; rb += 4) {
+      const int r0 = rb * 16;
+      v4d dr, di;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const cplx x = xc[(size_t)xj * cs + min(r0 + lk + 4 * v, n - 1)];
+        dr[v] = x.re;
+        di[v] = x.im;
+      }
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int kk = 4 * ks + lk, ra = r0 + li;
+        const cplx a0 = ld(A + (size_t)(lo + min(kk, bs - 1)) * n + min(ra, n - 1));
+        const cplx y0 = xc[(size_t)xj * cs + lo + min(kk, bs - 1)];
+        const bool ok = kk < bs && ra < lo;
+        const double ar = ok ? a0.re : 0.0, am = ok ? a0.im : 0.0;
+        const double yr = kk < bs ? y0.re : 0.0, ym = kk < bs ? y0.im : 0.0;
+        // z -= R y
+        dr = mfma4(-ar, yr, dr);
+        di = mfma4(-ar, ym, di);
+        dr = mfma4(am, ym, dr);
+        di = mfma4(-am, yr, di);
+      }
+      if (li < CB) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + lk + 4 * v;
+          if (row < lo) xc[(size_t)li * cs + row] = mk(dr[v], di[v]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = tid; r < n; r += 256) {
+    cplx v[CB];
+#pragma unroll
+    for (int q = 0; q < CB; ++q) v[q] = xc[(size_t)q * cs + r];
+#pragma unroll
+    for (int q = 0; q < CB; ++q)
+      if (q < ncol) st(B + (size_t)(j0 + q) * n + r, v[q]);
+  }
+}
+
+// k_form_Q_blocked on the matrix cores: CB columns of Q per workgroup from the identity, the
+// panels from the last one that reaches them down to the first (Q_p = I - V_p T_p V_p^H).
+template <int CB>
+__global__ __launch_bounds__(256) void k_form_Q_mfma(const ChainDesc* chains, MatRef Ar, MatRef Qr, int cs) {
+  int bx, by;
+  xcd_block(bx, by);
+  const ChainDesc& c = chains[by];
+  const int n = c.n;
+  const int j0 = bx * CB;
+  if (j0 >= n) return;
+  const int ncol = min(CB, n - j0);
+  const int tid = threadIdx.x;
+  const gcplx* A = gp(matptr(c, Ar));
+  gcplx* Q = gp(matptr(c, Qr));
+  extern __shared__ double smem[];
+  cplx* xc = reinterpret_cast<cplx*>(smem);  // [CB][cs]
+  cplx* V = xc + (size_t)CB * cs;             // [8][cs]
+  cplx* scr = V + (size_t)8 * cs;
+  __shared__ cplx Tl[64];
+  for (int e = tid; e < CB * n; e += 256) {
+    const int q = e / n, r = e - q * n;
+    xc[(size_t)q * cs + r] = mk(r == j0 + q ? 1.0 : 0.0, 0.0);
+  }
+  const int jmax = j0 + ncol - 1;  // panels starting beyond it leave these columns alone
+  for (int p = jmax / 8; p >= 0; --p) {
+    const int k0 = p * 8, kb = min(8, n - k0);
+    __syncthreads();
+    stage_panel<8>(V, A, c.qr_perm, k0, kb, n, tid, 1, cs);  // columns >= kb zero-filled
+    if (tid < 64) Tl[tid] = ld(gp(c.qr_Tf) + (size_t)p * 64 + tid);
+    __syncthreads();
+    apply_panel_mfma<CB, false>(xc, V, Tl, scr, n, cs, k0);
+  }
+  __syncthreads();
+  for (int e = tid; e < ncol * n; e += 256) {
+    const int q = e / n, r = e - q * n;
+    st(Q + (size_t)j0 * n + e, xc[(size_t)q * cs + r]);
+  }
+}
+
+// columns per workgroup of the matrix-core solve and Q formation at this size (0: the vector
+// kernels): 16 while X, V and the scratch fit the LDS (n <= 400), 8 up to n = 576
+inline int mfma_cols(int nmax) { return nmax <= 400 ? 16 : nmax <= 576 ? 8 : 0; }
+// LDS column stride of X and V: >= n and 1 mod 16, so that the 16 columns a quarter-wave reads
+// at one row (128-bit accesses) fall on 16 different groups of four banks
+inline int mfma_ld(int nmax) { return nmax + ((17 - (nmax & 15)) & 15); }
+inline size_t mfma_lds(int nmax, int cb) {
+  return ((size_t)(cb + 8) * mfma_ld(nmax) + kMfmaScratch) * sizeof(cplx);
+}
+
 // chain counts from which k_form_Q_blocked uses 8 or 16 columns per workgroup (k_qr_solve
 // keeps 4: its time per workgroup is the latency of the panel sequence, and wider blocks
 // measured slower)
@@ -703,12 +969,32 @@ hipError_t launch_form_Q_blocked_nb(hipStream_t s, const ChainDesc* chains, int 
   return hipGetLastError();
 }
 
+template <int CB>
+hipError_t launch_mfma_kernel(const void* fn, hipStream_t s, int nchains, int nmax, const ChainDesc* chains, MatRef A,
+                              MatRef X) {
+  const size_t lds = mfma_lds(nmax, CB);
+  {
+    hipError_t e = ensure_dynamic_lds(fn, lds);
+    if (e != hipSuccess) return e;
+  }
+  int cs = mfma_ld(nmax);
+  void* args[] = {(void*)&chains, (void*)&A, (void*)&X, (void*)&cs};
+  hipError_t e = hipLaunchKernel(fn, dim3((nmax + CB - 1) / CB, nchains), dim3(256), args, lds, s);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
+}
+
 hipError_t launch_form_Q_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef Q,
                                  int nb) {
   if (nmax > kQrMaxN || nmax < 16) return hipErrorInvalidValue;
   if (nb == 0) nb = qr_panel_width(nmax);
   if (nb == 16 && nmax <= 512) return launch_form_Q_blocked_nb<16, 4>(s, chains, nchains, nmax, A, Q);
   if (nb != 8) return hipErrorInvalidValue;
+  if (nchains >= kWideColChains) {
+    const int cb = mfma_cols(nmax);
+    if (cb == 16) return launch_mfma_kernel<16>((const void*)k_form_Q_mfma<16>, s, nchains, nmax, chains, A, Q);
+    if (cb == 8) return launch_mfma_kernel<8>((const void*)k_form_Q_mfma<8>, s, nchains, nmax, chains, A, Q);
+  }
   // many chains fill the chip with fewer, wider column blocks: each reads the reflectors once
   if (nchains >= kWideColChains && nmax <= 400) return launch_form_Q_blocked_nb<8, 16>(s, chains, nchains, nmax, A, Q);
   if (nchains >= kWideColChains && nmax <= 576) return launch_form_Q_blocked_nb<8, 8>(s, chains, nchains, nmax, A, Q);
@@ -721,6 +1007,11 @@ hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, 
   if (nb == 0) nb = qr_panel_width(nmax);
   if (nb == 16 && nmax <= 512) return launch_qr_solve_nb<16, 4>(s, chains, nchains, nmax, A, B);
   if (nb != 8) return hipErrorInvalidValue;
+  if (nchains >= kWideColChains) {
+    const int cb = mfma_cols(nmax);
+    if (cb == 16) return launch_mfma_kernel<16>((const void*)k_qr_solve_mfma<16>, s, nchains, nmax, chains, A, B);
+    if (cb == 8) return launch_mfma_kernel<8>((const void*)k_qr_solve_mfma<8>, s, nchains, nmax, chains, A, B);
+  }
   if (nmax <= 640) return launch_qr_solve_nb<8, 4>(s, chains, nchains, nmax, A, B);
   return launch_qr_solve_nb<8, 1>(s, chains, nchains, nmax, A, B);
 }
