@@ -25,6 +25,14 @@ namespace dqmc {
 constexpr int kQrMaxN = 1024;
 constexpr int kQrMaxNb = 16;
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+// one v_mfma_f64_16x16x4: c + A B for this lane's A(i, k) = a, B(k, j) = b
+__device__ inline v4d mfma4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+// LDS column stride of the panel V and of LDS-resident column blocks: >= n and 1 mod 16, so
+// that the 16 columns a quarter-wave reads at one row (128-bit accesses) fall on 16 different
+// groups of four banks
+inline int mfma_ld(int nmax) { return nmax + ((17 - (nmax & 15)) & 15); }
+
 __device__ inline cplx shfl_c(cplx v, int src) { return mk(__shfl(v.re, src, 64), __shfl(v.im, src, 64)); }
 __device__ inline cplx read_lane_c(cplx v, int lane) { return mk(read_lane(v.re, lane), read_lane(v.im, lane)); }
 
@@ -63,7 +71,8 @@ __device__ inline void stage_panel(cplx* P, const gcplx* A, const int* perm, int
 // Selections below are written without data-dependent branches around loads: a guard
 // around each LDS read makes the compiler wait for it before the next one.
 template <int NB, int RQ>
-__global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, MatRef Ar, VecRef Dr, int prepivot) {
+__global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, MatRef Ar, VecRef Dr, int prepivot,
+                                                    int ps, int mt) {
   constexpr int V = 2 * NB;               // doubles per butterfly
   constexpr int LS = NB == 8 ? 2 : 1;     // after it, lane l holds value l >> LS
   const ChainDesc& c = chains[blockIdx.y];
@@ -75,7 +84,8 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
   GroupSync g = chain_group(c);
 
   extern __shared__ double smem[];
-  cplx* P = reinterpret_cast<cplx*>(smem);  // [NB][n] panel V (rows k0..n-1 used)
+  cplx* P = reinterpret_cast<cplx*>(smem);  // [NB][ps] panel V (rows k0..n-1 used)
+  cplx* wscr = P + (size_t)NB * ps;          // [4][8][16] per-wave W of the matrix-core trailing update
   __shared__ int perm[kQrMaxN], lpos[kQrMaxN];
   __shared__ cplx Tl[NB * NB];  // Tl[j * NB + t] = T(j, t)
   __shared__ cplx taul[NB];
@@ -273,8 +283,8 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
         const cplx xv = x[q][t];
         if (r >= k0 && r < n) {
           const cplx pv = (t >= kb || r < k0 + t) ? mk(0, 0) : (r == k0 + t) ? mk(1, 0) : xv;
-          P[(size_t)t * n + r].re = pv.re;
-          P[(size_t)t * n + r].im = pv.im;
+          P[(size_t)t * ps + r].re = pv.re;
+          P[(size_t)t * ps + r].im = pv.im;
         }
         if (t < kb && r >= kw && r < n) {
           a[r].re = xv.re;
@@ -295,7 +305,7 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
       const int r = tid + 256 * q;
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
-        const cplx* src = P + (size_t)j * n + min(r, n - 1);
+        const cplx* src = P + (size_t)j * ps + min(r, n - 1);
         const cplx pv = mk(src->re, src->im);
         pj[q][j] = (r >= k0 && r < n) ? pv : mk(0, 0);
       }
@@ -363,27 +373,122 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
       }
     }
   };
+  const int ws = W == 1 ? 1 : W - 1, w0 = W == 1 ? 0 : 1;  // trailing workgroups, first one
+  // Trailing update of panel p (rows >= k0) on the matrix cores (NB = 8, mt set: many chains,
+  // few workgroups per chain, each with many columns), for workgroups w >= 1 of a group of
+  // W > 1: this workgroup's blocks (block b of the columns beyond k1 + NB goes to
+  // workgroup 1 + b mod (W-1)) in tiles of two blocks, 16 columns, tile q to wave q mod 4.
+  // Per tile: W = V^H X (K = rows, A(i, k) = conj V(r, i), B(k, j) = X(r, col_j)), then
+  // Z = T^H W with the vector units (lane (j, k) forms Z(k, j) and Z(k + 4, j), exactly its B
+  // operands of the second product), then X -= V Z in 16-row tiles. X streams from global
+  // memory twice (loads four steps ahead); V comes from P (stride ps: conflict-free).
+  auto trailing_mfma = [&](int k0, int k1) __attribute__((always_inline)) {
+    const int li = lane & 15, lk = lane >> 4, vi = min(li, NB - 1);
+    const int nbt = (n - k1 - NB + NB - 1) / NB;  // blocks beyond panel p+1
+    const int nbw = nbt > w - 1 ? (nbt - (w - 1) + ws - 1) / ws : 0;
+    const int nst = (n - k0) >> 2;
+    cplx* wsc = wscr + wave * 128;
+    for (int q = wave; 2 * q < nbw; q += nwaves) {
+      const int j = 2 * q + (li >> 3);
+      const int cl = k1 + NB + ((w - 1) + j * ws) * NB + (li & 7);
+      const bool valid = j < nbw && cl < n;
+      gcplx* a = A + (size_t)perm[valid ? cl : min(k1 + NB, n - 1)] * n;
+      v4d wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
+      cplx xb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xb[u] = ld(a + k0 + 4 * min(u, nst - 1) + lk);
+      for (int s0 = 0; s0 < nst; s0 += 4) {
+        cplx xc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xc[u] = xb[u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xb[u] = ld(a + k0 + 4 * min(s0 + 4 + u, nst - 1) + lk);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (s0 + u < nst) {
+            const cplx v = P[(size_t)vi * ps + k0 + 4 * (s0 + u) + lk];
+            const double vr = li < NB ? v.re : 0.0, vm = li < NB ? v.im : 0.0;
+            const double xr = valid ? xc[u].re : 0.0, xm = valid ? xc[u].im : 0.0;
+            // W += V^H X: re += Vr Xr + Vi Xi, im += Vr Xi - Vi Xr
+            wr = mfma4(vr, xr, wr);
+            wi = mfma4(vr, xm, wi);
+            wr = mfma4(vm, xm, wr);
+            wi = mfma4(-vm, xr, wi);
+          }
+        }
+      }
+      // lane (li, lk) holds W(lk, li) and W(lk + 4, li); every lane needs W(:, li)
+      wsc[lk * 16 + li] = mk(wr[0], wi[0]);
+      wsc[(lk + 4) * 16 + li] = mk(wr[1], wi[1]);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      cplx z0 = mk(0, 0), z1 = mk(0, 0);
+#pragma unroll
+      for (int l = 0; l < NB; ++l) {
+        const cplx wl = wsc[l * 16 + li];
+        cfma_cj(z0, Tl[l * NB + lk], wl);
+        cfma_cj(z1, Tl[l * NB + lk + 4], wl);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // W read before the next tile writes it
+      // X -= V Z over 16-row tiles: element v of lane (li, lk) is row r0 + lk + 4 v, column li
+      cplx cb[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cb[v] = ld(a + min(k0 + lk + 4 * v, n - 1));
+      for (int r0 = k0; r0 < n; r0 += 16) {
+        v4d dr, di;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          dr[v] = cb[v].re;
+          di[v] = cb[v].im;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cb[v] = ld(a + min(r0 + 16 + lk + 4 * v, n - 1));
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+          const int ra = r0 + li;
+          const cplx a0 = P[(size_t)(4 * ks + lk) * ps + min(ra, n - 1)];
+          const double ar = ra < n ? a0.re : 0.0, am = ra < n ? a0.im : 0.0;
+          const cplx z = ks ? z1 : z0;
+          // X -= V Z: re += -Vr Zr + Vi Zi, im += -Vr Zi - Vi Zr
+          dr = mfma4(-ar, z.re, dr);
+          di = mfma4(-ar, z.im, di);
+          dr = mfma4(am, z.im, dr);
+          di = mfma4(-am, z.re, di);
+        }
+        if (valid) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = r0 + lk + 4 * v;
+            if (row < n) {
+              a[row].re = dr[v];
+              a[row].im = di[v];
+            }
+          }
+        }
+      }
+    }
+  };
   // One-panel look-ahead: while every other workgroup applies panel p to the columns beyond
   // panel p+1 (in blocks of NB logical columns, block b to workgroup 1 + b mod (W-1)),
   // workgroup 0 applies it to panel p+1's columns and factors panel p+1, so each panel costs
   // one group barrier and its factorisation overlaps the trailing update. Workgroup 0 keeps
   // V and T of the panel it factored in P and Tl; the others stage them. Iteration p = -1
   // factors panel 0 alone. Each code path has one call site, for the instruction cache.
-  const int ws = W == 1 ? 1 : W - 1, w0 = W == 1 ? 0 : 1;
   for (int p = -1, k1 = 0; k1 < n; ++p, k1 += NB) {  // apply panel p, factor panel p+1 at k1
     const int k0 = k1 - NB;
     if (p >= 0) {
       if (w != 0) {
-        stage_panel<NB>(P, A, perm, k0, NB, n, tid, 1);
+        stage_panel<NB>(P, A, perm, k0, NB, n, tid, 1, ps);
         for (int e = tid; e < NB * NB; e += blockDim.x) Tl[e] = ld(gp(c.qr_Tf) + (size_t)p * NB * NB + e);
         __syncthreads();
       }
-      load_pj(k0);
+      if (w == 0 || NB != 8 || !mt) load_pj(k0);
     }
     // this workgroup's blocks: trailing ones first (kc >= k1 + NB), then, on workgroup 0,
     // panel p+1 itself, which stays in x for the factorisation
     if (p < 0) {
       if (w == 0) load_regs(0, min(NB, n), 0);
+    } else if (NB == 8 && w != 0 && mt) {
+      trailing_mfma(k0, k1);
     } else {
       for (int kc = (w >= w0) ? k1 + NB + (w - w0) * NB : n;; kc += ws * NB) {
         const bool next = kc >= n;
@@ -657,10 +762,7 @@ __global__ __launch_bounds__(256) void k_form_Q_blocked(const ChainDesc* chains,
 // B(k, j) on lane i + 16 k (resp. j + 16 k); D(i, j) element v on lane j + 16 (i - 4 v).
 // LDS: X [CB][ld], V [8][ld] (unit lower trapezoid from stage_panel), 3 x 8 x 16 scratch.
 // ---------------------------------------------------------------------------
-typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kMfmaScratch = 3 * 8 * 16;  // complex entries of the panel scratch
-
-__device__ inline v4d mfma4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 template <int CB, bool ADJ>
 __device__ inline void apply_panel_mfma(cplx* xc, const cplx* V, const cplx* Tl, cplx* scr, int n, int cs,
@@ -901,9 +1003,6 @@ __global__ __launch_bounds__(256) void k_form_Q_mfma(const ChainDesc* chains, Ma
 // columns per workgroup of the matrix-core solve and Q formation at this size (0: the vector
 // kernels): 16 while X, V and the scratch fit the LDS (n <= 400), 8 up to n = 576
 inline int mfma_cols(int nmax) { return nmax <= 400 ? 16 : nmax <= 576 ? 8 : 0; }
-// LDS column stride of X and V: >= n and 1 mod 16, so that the 16 columns a quarter-wave reads
-// at one row (128-bit accesses) fall on 16 different groups of four banks
-inline int mfma_ld(int nmax) { return nmax + ((17 - (nmax & 15)) & 15); }
 inline size_t mfma_lds(int nmax, int cb) {
   return ((size_t)(cb + 8) * mfma_ld(nmax) + kMfmaScratch) * sizeof(cplx);
 }
@@ -923,12 +1022,16 @@ int qr_panel_width(int nmax) {
 template <int NB, int RQ>
 hipError_t launch_qr_blocked_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
                                 int prepivot, int wg) {
-  const size_t lds = (size_t)NB * nmax * sizeof(cplx);
+  int ps = mfma_ld(nmax);
+  const size_t lds = ((size_t)NB * ps + 4 * 128) * sizeof(cplx);
   {
     hipError_t e = ensure_dynamic_lds((const void*)k_qr_blocked<NB, RQ>, lds);
     if (e != hipSuccess) return e;
   }
-  void* args[] = {(void*)&chains, (void*)&A, (void*)&D, (void*)&prepivot};
+  // the matrix-core trailing update pays off when each workgroup holds many trailing columns;
+  // with one chain its tiles are half empty and the block panel p+2 needs sits on one wave
+  int mt = nchains >= kWideColChains ? 1 : 0;
+  void* args[] = {(void*)&chains, (void*)&A, (void*)&D, (void*)&prepivot, (void*)&ps, (void*)&mt};
   hipError_t e =
       hipLaunchCooperativeKernel((const void*)k_qr_blocked<NB, RQ>, dim3(wg, nchains), dim3(256), args, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
