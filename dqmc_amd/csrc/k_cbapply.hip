@@ -203,16 +203,39 @@ __global__ __launch_bounds__(256) void k_apply_stages(const ChainDesc* chains, M
       evc[4 * i + 3] = R;
     }
   }
-  // load
-  if (!ROWS) {
-    for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
-      const int v = idx / n, k = idx - v * n;
-      vec[v * n + k] = ld(gp(X) + (size_t)(v0 + v) * n + k);
-    }
-  } else {
-    for (int idx = threadIdx.x; idx < nv * n; idx += blockDim.x) {
-      const int k = idx / nv, v = idx - k * nv;
-      vec[v * n + k] = ld(gp(X) + (size_t)k * n + (v0 + v));
+  // load: eight loads per thread in flight before their LDS stores (one memory latency per
+  // 2048 entries instead of one per 256)
+  {
+    const int tot = nv * n;
+    for (int i0 = threadIdx.x; i0 < tot; i0 += 8 * blockDim.x) {
+      cplx t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = min(i0 + u * (int)blockDim.x, tot - 1);
+        if (!ROWS) {
+          const int v = idx / n, k = idx - v * n;
+          t[u] = ld(gp(X) + (size_t)(v0 + v) * n + k);
+        } else {
+          const int k = idx / nv, v = idx - k * nv;
+          t[u] = ld(gp(X) + (size_t)k * n + (v0 + v));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = i0 + u * (int)blockDim.x;
+        if (idx < tot) {
+          int v, k;
+          if (!ROWS) {
+            v = idx / n;
+            k = idx - v * n;
+          } else {
+            k = idx / nv;
+            v = idx - k * nv;
+          }
+          vec[v * n + k].re = t[u].re;  // (whole-struct copies from a register array are
+          vec[v * n + k].im = t[u].im;  // lowered through a stack slot)
+        }
+      }
     }
   }
   __syncthreads();
