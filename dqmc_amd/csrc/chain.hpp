@@ -253,8 +253,10 @@ hipError_t launch_form_TQ(hipStream_t s, const ChainDesc* chains, int nchains, i
 // Blocked Householder QR of A (in place, k_qrp layout; D = |Re R_kk|); prepivot = 1 orders
 // the columns by decreasing initial norm first, 0 keeps them. nb: panel width, 16 (n <= 512)
 // or 8; 0 picks 16 up to n = 512, else 8.
+// bmap (nblocks entries): optional per-block (chain, workgroup, workgroup count) map of a 1-D
+// grid, chain | w << 16 | W << 24, for workgroup counts that differ between chains
 hipError_t launch_qr_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
-                             int prepivot, int wg, int nb = 0);
+                             int prepivot, int wg, int nb = 0, const int* bmap = nullptr, int nblocks = 0);
 // B <- R^-1 Q^H B with the factors of launch_qr_blocked(prepivot = 0) in A (same nb)
 hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef B,
                            int nb = 0);

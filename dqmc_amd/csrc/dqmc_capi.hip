@@ -97,6 +97,10 @@ struct Group {
                           // 4 windowed delayed updates
   int udt_route = 0;      // 0: blocked QR with pre-pivoting, 1: column-pivoted QR (zgeqp3 semantics)
   int qr_nb = 0;          // panel width of the blocked QR (0: the default, 8)
+  // merged sweeps: per-block map of the blocked QR's 1-D grid (workgroups per chain in
+  // proportion to n^3; launch_qr_blocked), nullptr for the uniform (wg_qr, chains) grid
+  const int* qr_map = nullptr;
+  int qr_map_blocks = 0;
   bool recomputed = false;
   // one sweep (M update steps) captured as a HIP graph per starting state and settings
   struct SweepGraph {
@@ -493,7 +497,7 @@ struct Ops {
     {
       ProfScope ps(g, DQMC_K_QRP);
       if (stack && g.udt_route == 0)
-        LCHK(launch_qr_blocked(s(), d(), nc(), g.n, A, D, 1, g.wg_qr, g.qr_nb));
+        LCHK(launch_qr_blocked(s(), d(), nc(), g.n, A, D, 1, g.wg_qr, g.qr_nb, g.qr_map, g.qr_map_blocks));
       else
         LCHK(launch_qrp(s(), d(), nc(), g.n, A, D, g.wg_qr));
     }
@@ -635,7 +639,8 @@ struct Ops {
       // [.] = Q R (Householder, no pivoting: the matrix is well conditioned), then
       // Y = [.]^-1 Dlb^-1 Ul^H = R^-1 Q^H (Dlb^-1 Ul^H); log|det [.]| = sum log |R_kk|
       ProfScope ps(g, DQMC_K_INVERSE);
-      LCHK(launch_qr_blocked(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D5), 0, g.wg_qr, g.qr_nb));
+      LCHK(launch_qr_blocked(s(), d(), nc(), g.n, NM(M_TMP), NV(V_D5), 0, g.wg_qr, g.qr_nb, g.qr_map,
+                             g.qr_map_blocks));
       LCHK(launch_adjoint(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_UL)));
       LCHK(launch_scale_rows_copy(s(), d(), nc(), g.n, NM(M_TMP2), NM(M_TMP2), NV(V_D1)));
       LCHK(launch_qr_solve(s(), d(), nc(), g.n, NM(M_TMP), NM(M_TMP2), g.qr_nb));
@@ -1283,14 +1288,49 @@ void plan_row(const Group& g, const StepRec& r, int* row) {
 constexpr int kMergedApplies = 4 + 16;  // apply launches of a step with a selector row: 2 + s + 2, s <= 16
 
 // Shape facts of a merged call and the layout of its plan buffer: per chain [T][kPlanRoles]
+// rows, then [T][kMergedApplies][nch] selectors and slices, then [T][nch] local-update slices,
+// then per active set the block map of the blocked QR.
+constexpr int kQrMapMax = 256;  // cooperative blocks of one QR launch: one per CU
+
+// Workgroups per chain for the blocked QR of one launch over chains of sizes n[k]: in
+// proportion to n^3 (the factorisation's work), at least 1, at most n / 8 and kMaxWG, together
+// at most kQrMapMax; the map lists chain | w << 16 | W << 24 per block.
+int qr_block_map(const std::vector<int>& n, std::vector<int>& map) {
+  const int na = (int)n.size();
+  double tot = 0.0;
+  for (int v : n) tot += (double)v * v * v;
+  std::vector<int> W(na);
+  int sum = 0;
+  for (int k = 0; k < na; ++k) {
+    const int cap = std::max(1, std::min(n[k] / 8, kMaxWG));
+    W[k] = std::max(1, std::min(cap, (int)std::floor(kQrMapMax * (double)n[k] * n[k] * n[k] / tot)));
+    sum += W[k];
+  }
+  while (sum > kQrMapMax) {  // the floors of 1 can overshoot: take from the largest
+    int kmax = 0;
+    for (int k = 1; k < na; ++k)
+      if (W[k] > W[kmax]) kmax = k;
+    if (W[kmax] <= 1) return -1;
+    --W[kmax];
+    --sum;
+  }
+  map.clear();
+  for (int k = 0; k < na; ++k)
+    for (int w = 0; w < W[k]; ++w) map.push_back(k | (w << 16) | (W[k] << 24));
+  return (int)map.size();
+}
+
+// Shape facts of a merged call and the layout of its plan buffer: per chain [T][kPlanRoles]
 // rows, then [T][kMergedApplies][nch] selectors and slices, then [T][nch] local-update slices.
 struct MergedShape {
   int s = 0, Mmax = 0, nch = 0;
   std::vector<int> Ms;                // distinct M, ascending
   std::vector<std::vector<int>> sets;  // set k: the chains with M >= Ms[k] (active while t < nsweeps Ms[k])
-  size_t ints(int T) const {
+  std::vector<std::vector<int>> qr_maps;  // set k: its QR block map (at qr_map_off(T) + k * kQrMapMax)
+  size_t qr_map_off(int T) const {
     return (size_t)nch * T * kPlanRoles + 2 * (size_t)T * kMergedApplies * nch + (size_t)T * nch;
   }
+  size_t ints(int T) const { return qr_map_off(T) + sets.size() * (size_t)kQrMapMax; }
 };
 
 MergedShape merged_shape(dqmc_batch* b) {
@@ -1308,6 +1348,10 @@ MergedShape merged_shape(dqmc_batch* b) {
     for (int ci = 0; ci < ms.nch; ++ci)
       if (b->chains[ci].M >= Mk) v.push_back(ci);
     ms.sets.push_back(v);
+    std::vector<int> ns, map;
+    for (int ci : v) ns.push_back(b->chains[ci].n);
+    if (qr_block_map(ns, map) < 0) map.clear();  // more chains than blocks: the uniform grid
+    ms.qr_maps.push_back(map);
   }
   return ms;
 }
@@ -1324,6 +1368,8 @@ int merged_plan(dqmc_batch* b, const MergedShape& ms, int nsweeps, long long t0,
   int* hloc = hsli + (size_t)T * kMergedApplies * nch;
   std::fill(hrows, hsel, -1);
   std::fill(hsel, hsli, -1);
+  for (size_t k = 0; k < ms.sets.size(); ++k)
+    std::copy(ms.qr_maps[k].begin(), ms.qr_maps[k].end(), h.begin() + ms.qr_map_off(T) + k * kQrMapMax);
   kinds.assign(T, 0);
   set_of.assign(T, 0);
   std::vector<StepRec> rec(ng);
@@ -1426,6 +1472,9 @@ int merged_enqueue(dqmc_batch* b, const MergedShape& ms, int T, const int* d_pla
       amin = std::min(amin, b->chains[ci].n);
     }
     mg.d_desc = const_cast<ChainDesc*>(d_desc) + set_off[set_of[t]];
+    const std::vector<int>& qmap = ms.qr_maps[set_of[t]];
+    mg.qr_map = qmap.empty() ? nullptr : d_plan + ms.qr_map_off(T) + (size_t)set_of[t] * kQrMapMax;
+    mg.qr_map_blocks = (int)qmap.size();
     mg.chains.assign(na, 0);
     mg.n = an;
     mg.N = aN;

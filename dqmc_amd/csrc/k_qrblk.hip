@@ -72,16 +72,23 @@ __device__ inline void stage_panel(cplx* P, const gcplx* A, const int* perm, int
 // around each LDS read makes the compiler wait for it before the next one.
 template <int NB, int RQ>
 __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, MatRef Ar, VecRef Dr, int prepivot,
-                                                    int ps, int mt) {
+                                                    int ps, const int* bmap) {
   constexpr int V = 2 * NB;               // doubles per butterfly
   constexpr int LS = NB == 8 ? 2 : 1;     // after it, lane l holds value l >> LS
-  const ChainDesc& c = chains[blockIdx.y];
+  // grid (W, chains), or with a block map a 1-D grid where block b serves chain bmap[b] & 0xFFFF
+  // as workgroup (bmap[b] >> 16) & 0xFF of (bmap[b] >> 24) (workgroup counts per chain)
+  const int bm = bmap ? bmap[blockIdx.x] : 0;
+  const ChainDesc& c = chains[bmap ? (bm & 0xFFFF) : blockIdx.y];
   const int n = c.n;
-  const int W = gridDim.x, w = blockIdx.x;
+  const int W = bmap ? (bm >> 24) & 0xFF : gridDim.x, w = bmap ? (bm >> 16) & 0xFF : blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   gcplx* A = gp(matptr(c, Ar));
   gdouble* D = gp(vecptr(c, Dr));
-  GroupSync g = chain_group(c);
+  const GroupSync g{c.bar_count, c.bar_gen, c.error, (unsigned int)W};
+  // the matrix-core trailing update pays off when each trailing workgroup holds many columns
+  // (at least 128 at the start); with few, its tiles are half empty and the block that panel
+  // p+2 needs sits on one wave
+  const bool mt = NB == 8 && W > 1 && n >= 128 * (W - 1);
 
   extern __shared__ double smem[];
   cplx* P = reinterpret_cast<cplx*>(smem);  // [NB][ps] panel V (rows k0..n-1 used)
@@ -374,9 +381,7 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
     }
   };
   const int ws = W == 1 ? 1 : W - 1, w0 = W == 1 ? 0 : 1;  // trailing workgroups, first one
-  // Trailing update of panel p (rows >= k0) on the matrix cores (NB = 8, mt set: many chains,
-  // few workgroups per chain, each with many columns), for workgroups w >= 1 of a group of
-  // W > 1: this workgroup's blocks (block b of the columns beyond k1 + NB goes to
+  // Trailing update of panel p (rows >= k0) on the matrix cores (mt), for workgroups w >= 1: this workgroup's blocks (block b of the columns beyond k1 + NB goes to
   // workgroup 1 + b mod (W-1)) in tiles of two blocks, 16 columns, tile q to wave q mod 4.
   // Per tile: W = V^H X (K = rows, A(i, k) = conj V(r, i), B(k, j) = X(r, col_j)), then
   // Z = T^H W with the vector units (lane (j, k) forms Z(k, j) and Z(k + 4, j), exactly its B
@@ -1021,30 +1026,32 @@ int qr_panel_width(int nmax) {
 
 template <int NB, int RQ>
 hipError_t launch_qr_blocked_nb(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
-                                int prepivot, int wg) {
+                                int prepivot, int wg, const int* bmap, int nblocks) {
   int ps = mfma_ld(nmax);
   const size_t lds = ((size_t)NB * ps + 4 * 128) * sizeof(cplx);
   {
     hipError_t e = ensure_dynamic_lds((const void*)k_qr_blocked<NB, RQ>, lds);
     if (e != hipSuccess) return e;
   }
-  // the matrix-core trailing update pays off when each workgroup holds many trailing columns;
-  // with one chain its tiles are half empty and the block panel p+2 needs sits on one wave
-  int mt = nchains >= kWideColChains ? 1 : 0;
-  void* args[] = {(void*)&chains, (void*)&A, (void*)&D, (void*)&prepivot, (void*)&ps, (void*)&mt};
-  hipError_t e =
-      hipLaunchCooperativeKernel((const void*)k_qr_blocked<NB, RQ>, dim3(wg, nchains), dim3(256), args, lds, s);
+  void* args[] = {(void*)&chains, (void*)&A, (void*)&D, (void*)&prepivot, (void*)&ps, (void*)&bmap};
+  const dim3 grid = bmap ? dim3(nblocks, 1) : dim3(wg, nchains);
+  hipError_t e = hipLaunchCooperativeKernel((const void*)k_qr_blocked<NB, RQ>, grid, dim3(256), args, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
   return e;
 }
 
 hipError_t launch_qr_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, VecRef D,
-                             int prepivot, int wg, int nb) {
+                             int prepivot, int wg, int nb, const int* bmap, int nblocks) {
   if (nmax > kQrMaxN) return hipErrorInvalidValue;
   if (nb == 0) nb = qr_panel_width(nmax);
-  if (nb == 16 && nmax <= 512) return launch_qr_blocked_nb<16, 2>(s, chains, nchains, nmax, A, D, prepivot, wg);
-  if (nb == 8 && nmax <= 512) return launch_qr_blocked_nb<8, 2>(s, chains, nchains, nmax, A, D, prepivot, wg);
-  if (nb == 8) return launch_qr_blocked_nb<8, 4>(s, chains, nchains, nmax, A, D, prepivot, wg);
+  if (nb == 16 && nmax <= 512)
+    return launch_qr_blocked_nb<16, 2>(s, chains, nchains, nmax, A, D, prepivot, wg, bmap, nblocks);
+  if (nb == 8 && nmax <= 512)
+    return launch_qr_blocked_nb<8, 2>(s, chains, nchains, nmax, A, D, prepivot, wg, bmap, nblocks);
+  // three rows per thread up to n = 768 (four spill the register-resident panel)
+  if (nb == 8 && nmax <= 768)
+    return launch_qr_blocked_nb<8, 3>(s, chains, nchains, nmax, A, D, prepivot, wg, bmap, nblocks);
+  if (nb == 8) return launch_qr_blocked_nb<8, 4>(s, chains, nchains, nmax, A, D, prepivot, wg, bmap, nblocks);
   return hipErrorInvalidValue;
 }
 
