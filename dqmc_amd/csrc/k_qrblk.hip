@@ -399,17 +399,18 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
       const bool valid = j < nbw && cl < n;
       gcplx* a = A + (size_t)perm[valid ? cl : min(k1 + NB, n - 1)] * n;
       v4d wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
-      cplx xb[4];
+      constexpr int PF = 8;  // k-steps of X loaded ahead of the products
+      cplx xb[PF];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xb[u] = ld(a + k0 + 4 * min(u, nst - 1) + lk);
-      for (int s0 = 0; s0 < nst; s0 += 4) {
-        cplx xc[4];
+      for (int u = 0; u < PF; ++u) xb[u] = ld(a + k0 + 4 * min(u, nst - 1) + lk);
+      for (int s0 = 0; s0 < nst; s0 += PF) {
+        cplx xc[PF];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xc[u] = xb[u];
+        for (int u = 0; u < PF; ++u) xc[u] = xb[u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xb[u] = ld(a + k0 + 4 * min(s0 + 4 + u, nst - 1) + lk);
+        for (int u = 0; u < PF; ++u) xb[u] = ld(a + k0 + 4 * min(s0 + PF + u, nst - 1) + lk);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < PF; ++u) {
           if (s0 + u < nst) {
             const cplx v = P[(size_t)vi * ps + k0 + 4 * (s0 + u) + lk];
             const double vr = li < NB ? v.re : 0.0, vm = li < NB ? v.im : 0.0;
@@ -435,18 +436,22 @@ __global__ __launch_bounds__(256) void k_qr_blocked(const ChainDesc* chains, Mat
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // W read before the next tile writes it
       // X -= V Z over 16-row tiles: element v of lane (li, lk) is row r0 + lk + 4 v, column li
-      cplx cb[4];
+      cplx cb[4], cn[4];  // the next two row tiles' X, loaded ahead
 #pragma unroll
-      for (int v = 0; v < 4; ++v) cb[v] = ld(a + min(k0 + lk + 4 * v, n - 1));
+      for (int v = 0; v < 4; ++v) {
+        cb[v] = ld(a + min(k0 + lk + 4 * v, n - 1));
+        cn[v] = ld(a + min(k0 + 16 + lk + 4 * v, n - 1));
+      }
       for (int r0 = k0; r0 < n; r0 += 16) {
         v4d dr, di;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           dr[v] = cb[v].re;
           di[v] = cb[v].im;
+          cb[v] = cn[v];
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) cb[v] = ld(a + min(r0 + 16 + lk + 4 * v, n - 1));
+        for (int v = 0; v < 4; ++v) cn[v] = ld(a + min(r0 + 32 + lk + 4 * v, n - 1));
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
           const int ra = r0 + li;
