@@ -305,14 +305,18 @@ int dqmc_batch_get_stats(dqmc_batch* b, int chain, dqmc_stats* st);
  * by the group's (current_slice, direction) at its start, the box and whether proposals are
  * logged, so one graph is captured per such state (two for alternating up/down sweeps) on
  * first use and replayed after; changing workgroup counts or routes drops the graphs. Same
- * results as eager launches; not used while kernel timing (dqmc_batch_profile) is on. */
+ * results as eager launches; not used while kernel timing (dqmc_batch_profile) is on.
+ * Merged sweeps (dqmc_batch_set_merged) are captured the same way, one merged sweep of every
+ * chain per start state of all groups, each graph with its own plan. */
 int dqmc_batch_use_graph(dqmc_batch* b, int enable);
 /* No reference counterpart. 1 (default): a batch of several shape groups (any L and M, same
  * safe_mult, checkerboard kind, all_checks, box and the default calculate_greens route) is
  * swept in lockstep, every kernel launched once for all chains: all chains recompute G on the
  * same steps (M is a multiple of safe_mult), and each chain's own stack elements, slices
  * and turning points come from a per-chain plan; a chain leaves the launches when its sweeps
- * are done. Same results as sweeping the groups one by one (0); ignores use_graph. */
+ * are done. Same results as sweeping the groups one by one (0) up to rounding in G (the
+ * workgroup counts per launch differ). The blocked QR's workgroups per chain follow the
+ * chain's size (in proportion to n^3). */
 int dqmc_batch_set_merged(dqmc_batch* b, int enable);
 int dqmc_use_graph(dqmc_ctx* ctx, int enable);
 /* Autotuner. No reference counterpart (the reference has one fixed CPU code path). Per
