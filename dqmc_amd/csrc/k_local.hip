@@ -933,7 +933,7 @@ __global__ __launch_bounds__(256) void k_local_updates_delayed(const ChainDesc* 
 // Same arithmetic as update_greens! (local_updates.jl:61-95) up to the order of sums.
 // ---------------------------------------------------------------------------
 constexpr int kWinMax = 12;     // window sites with LDS-resident own columns
-constexpr int kWinMaxGlob = 6;  // window sites otherwise (Bo and H of the own columns bound it)
+constexpr int kWinMaxGlob = 12;  // window sites otherwise (Bo and H of the own columns bound it)
 constexpr size_t kWindowLdsMax = 160 * 1024 - 1024;  // the CU's LDS less this kernel's static part
 
 __host__ __device__ inline int window_bo_stride(int n, int wg) { return (n + wg - 1) / wg + 1; }
@@ -1022,7 +1022,7 @@ __device__ inline void window_end_mfma(const gcplx* __restrict__ Gc, gcplx* __re
 
 template <bool TILE>
 __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* chains, int slice, const int* slices,
-                                                              double box, int dolog, int K) {
+                                                              double box, int dolog, int K, int lds_cap) {
   const ChainDesc& c = chains[blockIdx.y];
   const int n = c.n, N = c.N;
   const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
@@ -1031,6 +1031,14 @@ __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* c
   const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
   const int ncol = cb1 - cb0;
   const int ncs = window_bo_stride(n, W);
+  // K fits the largest chain of the launch; a smaller chain takes the largest window its own
+  // layout fits in the allocated LDS (fewer windows, fewer passes of G)
+  if (!TILE)
+    for (int k2 = kWinMaxGlob; k2 > K; --k2)
+      if (window_lds_bytes(n, N, W, k2, false) <= (size_t)lds_cap) {
+        K = k2;
+        break;
+      }
   const int K4 = 4 * K;
 
   extern __shared__ double smem[];
@@ -1338,16 +1346,20 @@ hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int ncha
                    : variant == 4 ? (tile ? (const void*)k_local_updates_window<true>
                                           : (const void*)k_local_updates_window<false>)
                                   : (const void*)k_local_updates;
+  // the window variant without LDS-resident columns takes the whole budget: its smaller
+  // chains size their windows to it
   const size_t lds = variant == 2   ? local_lds_tile(nmax, Nmax, wg)
                      : variant == 3 ? delayed_lds_bytes(nmax, Nmax)
-                     : variant == 4 ? window_lds_bytes(nmax, Nmax, wg, K, tile)
+                     : variant == 4 ? (tile ? window_lds_bytes(nmax, Nmax, wg, K, tile) : kWindowLdsMax)
                                     : local_lds_global(nmax, Nmax, wg);
   if (lds > (variant == 4 ? kWindowLdsMax : kLocalLdsMax)) return hipErrorInvalidValue;
   {
     hipError_t e = ensure_dynamic_lds(fn, lds);
     if (e != hipSuccess) return e;
   }
-  void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log, (void*)&K};
+  int lds_cap = (int)lds;
+  void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box,
+                  (void*)&log,    (void*)&K,     (void*)&lds_cap};
   hipError_t e = hipLaunchCooperativeKernel(fn, dim3(wg, nchains), dim3(256), args, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
   return e;
