@@ -5,6 +5,7 @@
 // is a batch of one. Matrices are n x n complex128, column-major (Julia
 // ComplexF64 layout), leading dimension n.
 #pragma once
+#include <vector>
 #include "device_common.hpp"
 
 namespace dqmc {
@@ -273,9 +274,13 @@ hipError_t launch_inverse_gather(hipStream_t s, const ChainDesc* chains, int nch
 // (needs local_tile_fits), 3 delays the rank-4 updates and flushes them in groups (needs
 // local_delayed_fits), 4 processes the sites in windows with one group barrier per window
 // (needs local_window_sites(nmax, wg) >= 2: the largest window that fits in LDS; *tile
-// tells whether the workgroups' own columns of G stay in LDS as well).
+// tells whether the workgroups' own columns of G stay in LDS as well). bmap (variant 4 only):
+// per-block (chain, workgroup, workgroup count) map of a 1-D grid of nblocks, from
+// local_block_map; each chain then sizes its own window.
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
-                                const int* slices, double box, int wg, int log, int variant);
+                                const int* slices, double box, int wg, int log, int variant,
+                                const int* bmap = nullptr, int nblocks = 0);
+int local_block_map(const std::vector<int>& n, std::vector<int>& map);
 bool local_tile_fits(int nmax, int wg);
 bool local_delayed_fits(int nmax);
 int local_window_sites(int nmax, int wg, bool* tile = nullptr);

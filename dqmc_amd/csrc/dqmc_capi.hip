@@ -101,6 +101,9 @@ struct Group {
   // proportion to n^3; launch_qr_blocked), nullptr for the uniform (wg_qr, chains) grid
   const int* qr_map = nullptr;
   int qr_map_blocks = 0;
+  // merged sweeps: the same for the window-variant local updates (local_block_map)
+  const int* local_map = nullptr;
+  int local_map_blocks = 0;
   bool recomputed = false;
   // one sweep (M update steps) captured as a HIP graph per starting state and settings
   struct SweepGraph {
@@ -1327,10 +1330,12 @@ struct MergedShape {
   std::vector<int> Ms;                // distinct M, ascending
   std::vector<std::vector<int>> sets;  // set k: the chains with M >= Ms[k] (active while t < nsweeps Ms[k])
   std::vector<std::vector<int>> qr_maps;  // set k: its QR block map (at qr_map_off(T) + k * kQrMapMax)
+  std::vector<std::vector<int>> local_maps;  // set k: local-update block map (after the QR maps)
   size_t qr_map_off(int T) const {
     return (size_t)nch * T * kPlanRoles + 2 * (size_t)T * kMergedApplies * nch + (size_t)T * nch;
   }
-  size_t ints(int T) const { return qr_map_off(T) + sets.size() * (size_t)kQrMapMax; }
+  size_t local_map_off(int T) const { return qr_map_off(T) + sets.size() * (size_t)kQrMapMax; }
+  size_t ints(int T) const { return local_map_off(T) + sets.size() * (size_t)kQrMapMax; }
 };
 
 MergedShape merged_shape(dqmc_batch* b) {
@@ -1352,6 +1357,8 @@ MergedShape merged_shape(dqmc_batch* b) {
     for (int ci : v) ns.push_back(b->chains[ci].n);
     if (qr_block_map(ns, map) < 0) map.clear();  // more chains than blocks: the uniform grid
     ms.qr_maps.push_back(map);
+    if (local_block_map(ns, map) < 0) map.clear();
+    ms.local_maps.push_back(map);
   }
   return ms;
 }
@@ -1368,8 +1375,10 @@ int merged_plan(dqmc_batch* b, const MergedShape& ms, int nsweeps, long long t0,
   int* hloc = hsli + (size_t)T * kMergedApplies * nch;
   std::fill(hrows, hsel, -1);
   std::fill(hsel, hsli, -1);
-  for (size_t k = 0; k < ms.sets.size(); ++k)
+  for (size_t k = 0; k < ms.sets.size(); ++k) {
     std::copy(ms.qr_maps[k].begin(), ms.qr_maps[k].end(), h.begin() + ms.qr_map_off(T) + k * kQrMapMax);
+    std::copy(ms.local_maps[k].begin(), ms.local_maps[k].end(), h.begin() + ms.local_map_off(T) + k * kQrMapMax);
+  }
   kinds.assign(T, 0);
   set_of.assign(T, 0);
   std::vector<StepRec> rec(ng);
@@ -1475,6 +1484,9 @@ int merged_enqueue(dqmc_batch* b, const MergedShape& ms, int T, const int* d_pla
     const std::vector<int>& qmap = ms.qr_maps[set_of[t]];
     mg.qr_map = qmap.empty() ? nullptr : d_plan + ms.qr_map_off(T) + (size_t)set_of[t] * kQrMapMax;
     mg.qr_map_blocks = (int)qmap.size();
+    const std::vector<int>& lmap = ms.local_maps[set_of[t]];
+    mg.local_map = lmap.empty() ? nullptr : d_plan + ms.local_map_off(T) + (size_t)set_of[t] * kQrMapMax;
+    mg.local_map_blocks = (int)lmap.size();
     mg.chains.assign(na, 0);
     mg.n = an;
     mg.N = aN;
@@ -1532,7 +1544,12 @@ int merged_enqueue(dqmc_batch* b, const MergedShape& ms, int T, const int* d_pla
       if (variant == 0)
         variant = local_window_sites(an, mg.wg_local) >= 2 ? 4 : local_tile_fits(an, mg.wg_local) ? 2
                   : local_delayed_fits(an) ? 3 : 1;
-      LCHK(launch_local_updates(mg.stream, dd, na, an, 0, d_loc + (size_t)t * nch, box, mg.wg_local, dolog, variant));
+      // without LDS-resident columns the chains of a launch get workgroups by size
+      bool tile = true;
+      if (variant == 4) local_window_sites(an, mg.wg_local, &tile);
+      const bool by_size = variant == 4 && !tile && mg.local_map;
+      LCHK(launch_local_updates(mg.stream, dd, na, an, 0, d_loc + (size_t)t * nch, box, mg.wg_local, dolog, variant,
+                                by_size ? mg.local_map : nullptr, by_size ? mg.local_map_blocks : 0));
     }
   }
   // timing records of the pseudo group belong to the first group (dqmc_batch_profile_read)

@@ -15,6 +15,8 @@
 // the updated G into the ping-pong partner buffer, so no workgroup ever writes
 // memory another is reading in the same step: one group barrier per accepted
 // proposal.
+#include <vector>
+
 #include "chain.hpp"
 
 namespace dqmc {
@@ -1022,12 +1024,18 @@ __device__ inline void window_end_mfma(const gcplx* __restrict__ Gc, gcplx* __re
 
 template <bool TILE>
 __global__ __launch_bounds__(256) void k_local_updates_window(const ChainDesc* chains, int slice, const int* slices,
-                                                              double box, int dolog, int K, int lds_cap) {
-  const ChainDesc& c = chains[blockIdx.y];
+                                                              double box, int dolog, int K, int lds_cap,
+                                                              const int* bmap) {
+  // grid (W, chains), or with a block map a 1-D grid where block b serves chain bmap[b] & 0xFFFF
+  // as workgroup (bmap[b] >> 16) & 0xFF of (bmap[b] >> 24) (workgroup counts per chain)
+  const int bm = bmap ? bmap[blockIdx.x] : 0;
+  const int ci = bmap ? (bm & 0xFFFF) : (int)blockIdx.y;
+  const ChainDesc& c = chains[ci];
   const int n = c.n, N = c.N;
-  const int W = gridDim.x, w = blockIdx.x, tid = threadIdx.x;
-  const int l = slices ? slices[blockIdx.y] : slice;
-  GroupSync g = chain_group(c);
+  const int W = bmap ? (bm >> 24) & 0xFF : (int)gridDim.x, w = bmap ? (bm >> 16) & 0xFF : (int)blockIdx.x;
+  const int tid = threadIdx.x;
+  const int l = slices ? slices[ci] : slice;
+  const GroupSync g{c.bar_count, c.bar_gen, c.error, (unsigned int)W};
   const int cb0 = (int)(((long long)n * w) / W), cb1 = (int)(((long long)n * (w + 1)) / W);
   const int ncol = cb1 - cb0;
   const int ncs = window_bo_stride(n, W);
@@ -1334,12 +1342,64 @@ constexpr size_t kLocalLdsMax = 150 * 1024;
 
 bool local_tile_fits(int nmax, int wg) { return local_lds_tile(nmax, nmax / 4, wg) <= kLocalLdsMax; }
 
+// Workgroups per chain of a window-variant launch without LDS-resident columns over chains of
+// sizes n[k] (a block map for launch_local_updates): every chain starts at the fewest
+// workgroups whose layout holds a window of 4 sites, then the chain whose workgroups each move
+// the most G per slice -- (windows per slice) x n^2 / W -- takes one more, until 256 (one per
+// CU) are given out; at most n / 4 and 255 per chain. -1 when the chains do not fit.
+int local_block_map(const std::vector<int>& n, std::vector<int>& map) {
+  const int na = (int)n.size();
+  auto kfit = [](int nn, int W) {
+    for (int K = kWinMaxGlob; K >= 2; --K)
+      if (window_lds_bytes(nn, nn / 4, W, K, false) <= kWindowLdsMax) return K;
+    return 0;
+  };
+  auto load = [&](int nn, int W) {
+    const int K = kfit(nn, W), N = nn / 4;
+    int R = (N + K - 1) / K;
+    R += R & 1;
+    return (double)R * nn * nn / W;
+  };
+  std::vector<int> W(na);
+  int sum = 0;
+  for (int k = 0; k < na; ++k) {
+    int w = 1;
+    while (w < 255 && kfit(n[k], w) < 4) ++w;
+    if (kfit(n[k], w) < 2) return -1;
+    W[k] = w;
+    sum += w;
+  }
+  if (sum > 256) return -1;
+  while (sum < 256) {
+    int best = -1;
+    double bl = 0.0;
+    for (int k = 0; k < na; ++k)
+      if (W[k] < std::min(255, n[k] / 4)) {
+        const double v = load(n[k], W[k]);
+        if (v > bl) {
+          bl = v;
+          best = k;
+        }
+      }
+    if (best < 0) break;
+    ++W[best];
+    ++sum;
+  }
+  map.clear();
+  for (int k = 0; k < na; ++k)
+    for (int w = 0; w < W[k]; ++w) map.push_back(k | (w << 16) | (W[k] << 24));
+  return (int)map.size();
+}
+
 hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, int slice,
-                                const int* slices, double box, int wg, int log, int variant) {
+                                const int* slices, double box, int wg, int log, int variant, const int* bmap,
+                                int nblocks) {
   const int Nmax = nmax / 4;
   if (variant < 1 || variant > 4) return hipErrorInvalidValue;
+  if (bmap && variant != 4) return hipErrorInvalidValue;
   bool tile = false;
-  int K = variant == 4 ? local_window_sites(nmax, wg, &tile) : 0;
+  // with a block map every chain sizes its own window (no LDS-resident columns)
+  int K = bmap ? 2 : variant == 4 ? local_window_sites(nmax, wg, &tile) : 0;
   if (variant == 4 && K < 2) return hipErrorInvalidValue;
   const void* fn = variant == 2   ? (const void*)k_local_updates_tile
                    : variant == 3 ? (const void*)k_local_updates_delayed
@@ -1358,9 +1418,11 @@ hipError_t launch_local_updates(hipStream_t s, const ChainDesc* chains, int ncha
     if (e != hipSuccess) return e;
   }
   int lds_cap = (int)lds;
-  void* args[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box,
-                  (void*)&log,    (void*)&K,     (void*)&lds_cap};
-  hipError_t e = hipLaunchCooperativeKernel(fn, dim3(wg, nchains), dim3(256), args, lds, s);
+  void* args1[] = {(void*)&chains, (void*)&slice, (void*)&slices, (void*)&box, (void*)&log, (void*)&K};
+  void* args4[] = {(void*)&chains, (void*)&slice, (void*)&slices,  (void*)&box,
+                   (void*)&log,    (void*)&K,     (void*)&lds_cap, (void*)&bmap};
+  const dim3 grid = bmap ? dim3(nblocks, 1) : dim3(wg, nchains);
+  hipError_t e = hipLaunchCooperativeKernel(fn, grid, dim3(256), variant == 4 ? args4 : args1, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
   return e;
 }

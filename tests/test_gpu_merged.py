@@ -94,6 +94,30 @@ def test_merged_graph_replay_equals_eager(amd):
     bg.close()
 
 
+def test_merged_workgroups_by_size_equal_groupwise(amd):
+    """16 chains of L=10 and L=12 (n = 400, 576): their columns of G do not fit in LDS at the
+    merged launch's workgroup counts, so the merged local updates and QR give each chain
+    workgroups by size (a block map over a 1-D cooperative grid) and each chain sizes its own
+    window. Same decisions as sweeping the two shape groups one by one, G to rounding."""
+    mod = amd
+    ps = [config_params(mod, 10 if k % 2 == 0 else 12, 20, lam=0.8 + 0.05 * k) for k in range(16)]
+    _say("merged, workgroups by size: 8 x L=10 and 8 x L=12, M=20, 2 sweeps")
+    bm, _ = _batch(mod, ps, True, 3600)
+    bg, _ = _batch(mod, ps, False, 3600)
+    for b in (bm, bg):
+        b.sweep(2)
+    for k, p in enumerate(ps):
+        sm, sg = bm.stats(k), bg.stats(k)
+        assert sm.device_error == 0, k
+        assert sm.accepted == sg.accepted and sm.proposed == sg.proposed == 2 * p.N * p.slices, k
+        assert bm.streams[k].consumed == bg.streams[k].consumed, k
+        assert np.array_equal(bm.get_field(k), bg.get_field(k)), k
+        assert np.abs(bm.greens(k) - bg.greens(k)).max() < 1e-11, k
+        assert sm.drift_max < 1e-8, k
+    bm.close()
+    bg.close()
+
+
 def test_merged_chain_matches_oracle(amd):
     """A chain of a merged batch (L=6, beta=2, with L=4, beta=4 and L=8, beta=3 beside it)
     reproduces the oracle's field, G and stream consumption after 3 sweeps."""
@@ -119,11 +143,13 @@ def test_merged_chain_matches_oracle(amd):
 
 @pytest.mark.parametrize("L,M", [(6, 20), (12, 10)])
 def test_wide_column_kernels_match_single_chains(amd, L, M):
-    """From 8 chains per launch the Q formation gives each workgroup 16 (n <= 400) or 8 (n <= 576)
-    columns instead of 4; a 9-chain batch must still equal single chains."""
+    """From 8 chains per launch the QR solve and the Q formation run on the matrix cores with 16
+    (n <= 400) or 8 (n <= 576) LDS-resident columns per workgroup, and the blocked QR's trailing
+    update on the matrix cores where each trailing workgroup holds enough columns; a 9-chain
+    batch must still equal single chains (the vector kernels)."""
     mod = amd
     from dqmc_amd import DQMC, Batch, UniformStream
-    _say("wide-column Q formation: 9 chains L=%d M=%d vs single chains" % (L, M))
+    _say("matrix-core QR kernels: 9 chains L=%d M=%d vs single chains" % (L, M))
     ps = [config_params(mod, L, M, lam=0.8 + 0.1 * k) for k in range(9)]
     confs = [np.random.default_rng(3300 + k).uniform(-1, 1, (3, p.N, p.slices)) for k, p in enumerate(ps)]
     b = Batch(ps, streams=[UniformStream(3400 + k) for k in range(9)])
