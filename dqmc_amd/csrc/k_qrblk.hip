@@ -869,13 +869,13 @@ __device__ inline void apply_panel_mfma(cplx* xc, const cplx* V, const cplx* Tl,
 // column; the block's inverse diagonal precomputed once), then the rows above take the
 // block's contribution as MFMA tiles, R(r, lo:hi) y(lo:hi, :) (K = 16).
 template <int CB>
-__global__ __launch_bounds__(256) void k_qr_solve_mfma(const ChainDesc* chains, MatRef Ar, MatRef Br, int cs) {
+__global__ __launch_bounds__(256) void k_qr_solve_mfma(const ChainDesc* chains, MatRef Ar, MatRef Br, int cs, int nlo, int nhi) {
   int bx, by;
   xcd_block(bx, by);
   const ChainDesc& c = chains[by];
   const int n = c.n;
   const int j0 = bx * CB;
-  if (j0 >= n) return;
+  if (j0 >= n || n <= nlo || n > nhi) return;  // outside this launch's size class
   const int ncol = min(CB, n - j0);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, li = lane & 15, lk = lane >> 4;
   const gcplx* A = gp(matptr(c, Ar));
@@ -974,13 +974,13 @@ __global__ __launch_bounds__(256) void k_qr_solve_mfma(const ChainDesc* chains, 
 // k_form_Q_blocked on the matrix cores: CB columns of Q per workgroup from the identity, the
 // panels from the last one that reaches them down to the first (Q_p = I - V_p T_p V_p^H).
 template <int CB>
-__global__ __launch_bounds__(256) void k_form_Q_mfma(const ChainDesc* chains, MatRef Ar, MatRef Qr, int cs) {
+__global__ __launch_bounds__(256) void k_form_Q_mfma(const ChainDesc* chains, MatRef Ar, MatRef Qr, int cs, int nlo, int nhi) {
   int bx, by;
   xcd_block(bx, by);
   const ChainDesc& c = chains[by];
   const int n = c.n;
   const int j0 = bx * CB;
-  if (j0 >= n) return;
+  if (j0 >= n || n <= nlo || n > nhi) return;  // outside this launch's size class
   const int ncol = min(CB, n - j0);
   const int tid = threadIdx.x;
   const gcplx* A = gp(matptr(c, Ar));
@@ -1086,17 +1086,29 @@ hipError_t launch_form_Q_blocked_nb(hipStream_t s, const ChainDesc* chains, int 
 
 template <int CB>
 hipError_t launch_mfma_kernel(const void* fn, hipStream_t s, int nchains, int nmax, const ChainDesc* chains, MatRef A,
-                              MatRef X) {
+                              MatRef X, int nlo, int nhi) {
   const size_t lds = mfma_lds(nmax, CB);
   {
     hipError_t e = ensure_dynamic_lds(fn, lds);
     if (e != hipSuccess) return e;
   }
   int cs = mfma_ld(nmax);
-  void* args[] = {(void*)&chains, (void*)&A, (void*)&X, (void*)&cs};
+  void* args[] = {(void*)&chains, (void*)&A, (void*)&X, (void*)&cs, (void*)&nlo, (void*)&nhi};
   hipError_t e = hipLaunchKernel(fn, dim3((nmax + CB - 1) / CB, nchains), dim3(256), args, lds, s);
   if (e != hipSuccess) (void)hipGetLastError();
   return e;
+}
+
+// the matrix-core solve / Q formation over chains up to nmax: chains with n <= 400 take 16
+// columns per workgroup, larger ones 8 (two launches when the sizes are mixed; the chains of
+// the other class return at once)
+template <class F16, class F8>
+hipError_t launch_mfma_classes(F16 k16, F8 k8, hipStream_t s, int nchains, int nmax, const ChainDesc* chains, MatRef A,
+                               MatRef X) {
+  if (nmax <= 400) return launch_mfma_kernel<16>((const void*)k16, s, nchains, nmax, chains, A, X, 0, nmax);
+  hipError_t e = launch_mfma_kernel<16>((const void*)k16, s, nchains, 400, chains, A, X, 0, 400);
+  if (e != hipSuccess) return e;
+  return launch_mfma_kernel<8>((const void*)k8, s, nchains, nmax, chains, A, X, 400, nmax);
 }
 
 hipError_t launch_form_Q_blocked(hipStream_t s, const ChainDesc* chains, int nchains, int nmax, MatRef A, MatRef Q,
@@ -1107,8 +1119,7 @@ hipError_t launch_form_Q_blocked(hipStream_t s, const ChainDesc* chains, int nch
   if (nb != 8) return hipErrorInvalidValue;
   if (nchains >= kWideColChains) {
     const int cb = mfma_cols(nmax);
-    if (cb == 16) return launch_mfma_kernel<16>((const void*)k_form_Q_mfma<16>, s, nchains, nmax, chains, A, Q);
-    if (cb == 8) return launch_mfma_kernel<8>((const void*)k_form_Q_mfma<8>, s, nchains, nmax, chains, A, Q);
+    if (cb) return launch_mfma_classes(k_form_Q_mfma<16>, k_form_Q_mfma<8>, s, nchains, nmax, chains, A, Q);
   }
   // many chains fill the chip with fewer, wider column blocks: each reads the reflectors once
   if (nchains >= kWideColChains && nmax <= 400) return launch_form_Q_blocked_nb<8, 16>(s, chains, nchains, nmax, A, Q);
@@ -1124,8 +1135,7 @@ hipError_t launch_qr_solve(hipStream_t s, const ChainDesc* chains, int nchains, 
   if (nb != 8) return hipErrorInvalidValue;
   if (nchains >= kWideColChains) {
     const int cb = mfma_cols(nmax);
-    if (cb == 16) return launch_mfma_kernel<16>((const void*)k_qr_solve_mfma<16>, s, nchains, nmax, chains, A, B);
-    if (cb == 8) return launch_mfma_kernel<8>((const void*)k_qr_solve_mfma<8>, s, nchains, nmax, chains, A, B);
+    if (cb) return launch_mfma_classes(k_qr_solve_mfma<16>, k_qr_solve_mfma<8>, s, nchains, nmax, chains, A, B);
   }
   if (nmax <= 640) return launch_qr_solve_nb<8, 4>(s, chains, nchains, nmax, A, B);
   return launch_qr_solve_nb<8, 1>(s, chains, nchains, nmax, A, B);
