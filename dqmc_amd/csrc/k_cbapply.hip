@@ -303,9 +303,12 @@ static hipError_t run_stages(hipStream_t s, const ChainDesc* chains, int nchains
                              bool rows, const StageList& sl, int slice, const int* slices,
                              const StageList* sl1 = nullptr, const int* sel = nullptr) {
   const StageList& s1 = sl1 ? *sl1 : sl;
-  // tile of vectors per block: enough blocks to cover the chip, LDS <= 64 KB
+  // tile of vectors per block: LDS <= 64 KB, and narrower tiles until the launch has a block
+  // per CU (a lone chain at n = 400 would otherwise run 100 / 50 blocks); rows stay >= 2 wide
+  // (32-byte pieces of each column)
   int vpb = rows ? 8 : 4;
   while (vpb > 1 && (size_t)vpb * nmax * 16 > 65536) vpb >>= 1;
+  while (vpb > (rows ? 2 : 1) && (long long)((nmax + vpb - 1) / vpb) * nchains < 256) vpb >>= 1;
   const size_t lds = (size_t)vpb * nmax * 16 + (size_t)Nmax * 32;
   dim3 grid((nmax + vpb - 1) / vpb, nchains);
   {
