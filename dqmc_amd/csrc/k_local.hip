@@ -964,15 +964,6 @@ __device__ inline void window_end_mfma(const gcplx* __restrict__ Gc, gcplx* __re
   const int nrb = (n + 15) >> 4, ncp = (ncol + 31) >> 5;  // row blocks, pairs of column tiles
   for (int rb = wave; rb < nrb; rb += nw) {
     const int row = rb * 16 + li, rr = min(row, n - 1);
-    double yr[kWinMaxGlob], yi[kWinMaxGlob];
-#pragma unroll
-    for (int b = 0; b < kWinMaxGlob; ++b) {
-      const int col = accs[min(b, m - 1)] + lk * N;
-      cplx v = ld(Gc + (size_t)col * n + rr);
-      if (rr == col) v.re -= 1.0;
-      yr[b] = b < m ? v.re : 0.0;
-      yi[b] = b < m ? v.im : 0.0;
-    }
     // G tile loads of a pair of column tiles starting at own column c0
     auto load_pair = [&](int c0, cplx (&g)[8]) __attribute__((always_inline)) {
 #pragma unroll
@@ -981,8 +972,24 @@ __device__ inline void window_end_mfma(const gcplx* __restrict__ Gc, gcplx* __re
         g[v] = ld(Gc + (size_t)(cb0 + col) * n + rr);
       }
     };
-    cplx nxt[8];
+    // the first two pairs' loads go out before the Yt fragments', and every pair's loads two
+    // pairs ahead of its products
+    cplx nxt[8], nx2[8];
     load_pair(0, nxt);
+    load_pair(min(32, (ncp - 1) * 32), nx2);
+    double yr[kWinMaxGlob], yi[kWinMaxGlob];
+#pragma unroll
+    for (int b = 0; b < kWinMaxGlob; ++b) {
+      yr[b] = 0.0;
+      yi[b] = 0.0;
+      if (b < m) {  // wave-uniform
+        const int col = accs[b] + lk * N;
+        cplx v = ld(Gc + (size_t)col * n + rr);
+        if (rr == col) v.re -= 1.0;
+        yr[b] = v.re;
+        yi[b] = v.im;
+      }
+    }
     for (int cp = 0; cp < ncp; ++cp) {
       const int c0 = cp * 32;
       v4d ar0, ai0, ar1, ai1;
@@ -993,7 +1000,9 @@ __device__ inline void window_end_mfma(const gcplx* __restrict__ Gc, gcplx* __re
         ar1[v] = nxt[v + 4].re;
         ai1[v] = nxt[v + 4].im;
       }
-      if (cp + 1 < ncp) load_pair(c0 + 32, nxt);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) nxt[v] = nx2[v];
+      if (cp + 2 < ncp) load_pair(c0 + 64, nx2);
       const int w0 = min(c0 + li, ncol - 1), w1 = min(c0 + 16 + li, ncol - 1);
 #pragma unroll
       for (int b = 0; b < kWinMaxGlob; ++b) {
