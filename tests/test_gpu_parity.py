@@ -429,7 +429,7 @@ def test_local_update_variants(amd, variant):
 
 
 def test_seeded_chain_dense(amd):
-    """configs[0] model (L=4, no checkerboard) for 2 sweeps at beta=5."""
+    """The dense model (L=4, no checkerboard) for 2 sweeps at beta=5."""
     mod, DQMC = amd
     _say("seeded chain L=4 M=50 dense (CBFalse), 2 sweeps")
     p = config_params(mod, 4, 50, checkerboard=False)
@@ -439,7 +439,7 @@ def test_seeded_chain_dense(amd):
     c.close()
 
 
-def test_config1_size_parity(amd):
+def test_L10_beta20_size_parity(amd):
     """L=10, beta=20 (n=400, M=200): G after init and after one sweep vs the oracle."""
     mod, DQMC = amd
     _say("L=10 M=200 init + 1 sweep vs oracle (oracle ~1 min on the host)")
@@ -505,8 +505,8 @@ def test_batch_matches_single_chains(amd):
 
 
 def test_hmc_trajectory_matches_cpu_leapfrog(amd):
-    """configs[3]-style trajectory (10 leapfrog steps, eps 0.02) on a small chain vs a CPU
-    leapfrog driven by the oracle's trace-formula gradient."""
+    """An HMC trajectory of the gradient workload's kind (10 leapfrog steps, eps 0.02) on a
+    small chain vs a CPU leapfrog driven by the oracle's trace-formula gradient."""
     mod, DQMC = amd
     _say("HMC trajectory L=4 M=20 vs CPU leapfrog")
     p = config_params(mod, 4, 20)

@@ -41,7 +41,7 @@ def amd():
     return dqmc_amd, DQMC
 
 
-def test_config0_ten_sweeps(amd):
+def test_dense_L4_beta5_ten_sweeps(amd):
     """L=4, beta=5, no checkerboard (CBFalse): 10 sweeps, the oracle's decisions."""
     mod, DQMC = amd
     _say("L=4 M=50 dense, 10 sweeps vs oracle")
@@ -54,7 +54,7 @@ def test_config0_ten_sweeps(amd):
     c.close()
 
 
-def test_config1_two_sweeps(amd):
+def test_L10_beta20_two_sweeps(amd):
     """L=10, beta=20 (n=400, M=200), checkerboard: 2 sweeps with identical decisions and G
     within 1e-10 of the oracle after each (the oracle takes ~25 s per sweep on the host)."""
     mod, DQMC = amd
@@ -85,7 +85,7 @@ def mixed_batch_params(mod, nchains=64):
     return ps
 
 
-def test_config2_mixed_batch(amd):
+def test_mixed_batch_64_chains(amd):
     """64 chains of 12 shapes advanced together for 2 sweeps; every chain's drift below 1e-8
     and G against a direct recompute; the six L=6, beta=5 chains (c = 0, 12, ..., 60) against
     the oracle: the same field after 2 sweeps (so the same decisions) and G within 1e-10."""
@@ -116,7 +116,7 @@ def test_config2_mixed_batch(amd):
     b.close()
 
 
-def test_config3_gradient(amd):
+def test_gradient_L12_beta10(amd):
     """L=12, beta=10 (n=576, M=100): dS/dphi for all 43,200 components.
 
     Central differences use the device's own S = S_B + log|det G| (dqmc_grad's S output, the
@@ -164,7 +164,7 @@ def test_config3_gradient(amd):
     c.close()
 
 
-def test_config3_hmc(amd):
+def test_hmc_L12_beta10(amd):
     """HMC at L=12, beta=10: two trajectories of 10 leapfrog steps (eps 0.02). Each equals a
     host leapfrog driven by the device gradient (field within 1e-9, dH within 1e-6), and halving
     the step (20 steps of 0.01) cuts |dH| by about 4 (second-order integrator)."""
@@ -211,7 +211,7 @@ def test_config3_hmc(amd):
     c.close()
 
 
-def test_config4_sweep_tdgf_hmc(amd):
+def test_L16_beta40_sweep_tdgf_hmc(amd):
     """L=16, beta=40 (n=1024, M=400), checkerboard, safe_mult 10."""
     mod, DQMC = amd
     from dqmc_amd import UniformStream
