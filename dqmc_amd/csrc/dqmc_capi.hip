@@ -1295,27 +1295,34 @@ constexpr int kMergedApplies = 4 + 16;  // apply launches of a step with a selec
 // then per active set the block map of the blocked QR.
 constexpr int kQrMapMax = 256;  // cooperative blocks of one QR launch: one per CU
 
-// Workgroups per chain for the blocked QR of one launch over chains of sizes n[k]: in
-// proportion to n^3 (the factorisation's work), at least 1, at most n / 8 and kMaxWG, together
-// at most kQrMapMax; the map lists chain | w << 16 | W << 24 per block.
+// Workgroups per chain for the blocked QR of one launch over chains of sizes n[k]: every chain
+// starts at 1, then the chain with the longest estimated factorisation takes one more until
+// kQrMapMax are given out (at most n / 8 and kMaxWG each). Estimate, fitted to single-chain
+// timings on MI355X at n = 256 ... 576: t(n, W) = c(n) + d(n) / (W - 1) for W >= 2 (the panel
+// chain of workgroup 0, and the trailing update shared by the other W - 1), c ~ n^1.24,
+// d ~ n^3.83; with W = 1 the one workgroup does both, 2.26 (c + d). The map lists
+// chain | w << 16 | W << 24 per block.
 int qr_block_map(const std::vector<int>& n, std::vector<int>& map) {
   const int na = (int)n.size();
-  double tot = 0.0;
-  for (int v : n) tot += (double)v * v * v;
-  std::vector<int> W(na);
-  int sum = 0;
-  for (int k = 0; k < na; ++k) {
-    const int cap = std::max(1, std::min(n[k] / 8, kMaxWG));
-    W[k] = std::max(1, std::min(cap, (int)std::floor(kQrMapMax * (double)n[k] * n[k] * n[k] / tot)));
-    sum += W[k];
-  }
-  while (sum > kQrMapMax) {  // the floors of 1 can overshoot: take from the largest
-    int kmax = 0;
-    for (int k = 1; k < na; ++k)
-      if (W[k] > W[kmax]) kmax = k;
-    if (W[kmax] <= 1) return -1;
-    --W[kmax];
-    --sum;
+  if (na > kQrMapMax) return -1;
+  auto est = [](int nn, int W) {
+    const double x = nn / 400.0, c = 0.7 * std::pow(x, 1.24), d = 4.2 * std::pow(x, 3.83);
+    return W >= 2 ? c + d / (W - 1) : 2.26 * (c + d);
+  };
+  std::vector<int> W(na, 1);
+  for (int sum = na; sum < kQrMapMax; ++sum) {
+    int best = -1;
+    double bt = 0.0;
+    for (int k = 0; k < na; ++k)
+      if (W[k] < std::max(1, std::min(n[k] / 8, kMaxWG))) {
+        const double t = est(n[k], W[k]);
+        if (t > bt) {
+          bt = t;
+          best = k;
+        }
+      }
+    if (best < 0) break;
+    ++W[best];
   }
   map.clear();
   for (int k = 0; k < na; ++k)

@@ -315,8 +315,8 @@ int dqmc_batch_use_graph(dqmc_batch* b, int enable);
  * same steps (M is a multiple of safe_mult), and each chain's own stack elements, slices
  * and turning points come from a per-chain plan; a chain leaves the launches when its sweeps
  * are done. Same results as sweeping the groups one by one (0) up to rounding in G (the
- * workgroup counts per launch differ). The blocked QR's workgroups per chain follow the
- * chain's size (in proportion to n^3). */
+ * workgroup counts per launch differ). The blocked QR's and the local updates' workgroups
+ * per chain follow the chain's size. */
 int dqmc_batch_set_merged(dqmc_batch* b, int enable);
 int dqmc_use_graph(dqmc_ctx* ctx, int enable);
 /* Autotuner. No reference counterpart (the reference has one fixed CPU code path). Per
