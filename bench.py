@@ -192,22 +192,55 @@ def batch_measure(amd, params, sweeps, warmup, tune=True, roofline=True):
 
 def grad_measure(amd, evals, warmup):
     """dS/dphi evaluations per second at L=12, beta=10 (n=576, M=100): one stabilised pass over
-    all slices with the trace formula at every (component, site, slice)."""
-    from dqmc_amd import DQMC, UniformStream
+    all slices with the trace formula at every (component, site, slice).
+
+    The warm-up evaluations run with every kernel class timed by HIP events on the chain's
+    stream; they give the dominant class, its average launch time (the roofline entry, with
+    the flop/byte model of algorithmic_work) and the evaluation's algorithmic FP64 work: the
+    flop-bound classes' launches per evaluation times their model work. The timed
+    evaluations run without events."""
+    from dqmc_amd import DQMC, UniformStream, _capi
     p = workload_params(amd, L=12, slices=100, seed=3001)
     phi = np.random.default_rng(p.seed).uniform(-1.0, 1.0, (3, p.N, p.slices))
     c = DQMC(p, uniforms=UniformStream(p.seed))
     c.init(phi)
+    warmup = max(warmup, 1)
+    ms = np.zeros(_capi.K_NCLASS)
+    cnt = np.zeros(_capi.K_NCLASS, dtype=np.int64)
+    _capi.check(c.lib.dqmc_profile_read(c.h, None, None, 0))  # drop the init's records
+    c.lib.dqmc_profile(c.h, (1 << _capi.K_NCLASS) - 1)
     for _ in range(warmup):
         c.grad()
+    _capi.check(c.lib.dqmc_profile_read(c.h, _capi.dptr(ms), cnt.ctypes.data_as(_capi._LL), _capi.K_NCLASS))
+    c.lib.dqmc_profile(c.h, 0)
     c.synchronize()
     t0 = time.perf_counter()
     for _ in range(evals):
         g, _s = c.grad()
     dt = time.perf_counter() - t0
     c.close()
+    dom = int(np.argmax(ms))
+    avg_ms = float(ms[dom]) / max(int(cnt[dom]), 1)
+    bound, work, unit = algorithmic_work(dom, p.n, 0.0)
+    if bound == "hbm":
+        achieved, peak, runit = work / (avg_ms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s"
+    else:
+        achieved, peak, runit = work / (avg_ms * 1e-3) / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
+    flop = 0.0
+    for k in range(_capi.K_NCLASS):
+        b, w, _u = algorithmic_work(k, p.n, 0.0)
+        if b == "mfma" and cnt[k]:
+            flop += w * float(cnt[k]) / warmup
+    eval_tflops = flop * evals / dt / 1e12
     return {"value": evals / dt, "unit": "evals/s", "config": "L=12 beta=10 dtau=0.1 (M=100), n=576",
-            "components": int(g.size), "evals": evals, "seconds": dt}
+            "components": int(g.size), "evals": evals, "seconds": dt,
+            "roofline": {"bound": bound, "kernel": _capi.K_NAMES[dom], "achieved": achieved, "peak": peak,
+                         "unit": runit, "frac": achieved / peak, "work_per_launch": work, "work_unit": unit,
+                         "avg_launch_ms": avg_ms, "launches_per_eval": int(cnt[dom]) // warmup},
+            "eval_flop": {"flop_per_eval": flop, "achieved": eval_tflops, "peak": PEAK_FP64_TFLOPS,
+                          "unit": "TFLOP/s", "frac": eval_tflops / PEAK_FP64_TFLOPS},
+            "kernel_ms_per_eval_warmup": {_capi.K_NAMES[k]: round(float(ms[k]) / warmup, 3)
+                                          for k in range(_capi.K_NCLASS)}}
 
 
 def calibration_copy(mib):
